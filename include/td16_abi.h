@@ -1,0 +1,273 @@
+/*
+ * td16 witness search -- C ABI of libtd16.so (MI355X / gfx950).
+ *
+ * Drop-in boundary for the reference's JNI search library `td16_jni`
+ * (reference: emitter/src/native/jni/jni_sorter.cu).  Every entry point takes
+ * plain pointers and sizes; no HIP or torch types cross this boundary.  A JNI
+ * shim that exports the reference's exact `Java_br_unb_cic_tdp_base_
+ * GPUSortingSearch_*` names on top of these functions is in
+ * td16_amd/csrc/jni_sorter.cpp.
+ *
+ * Conventions
+ *  - Symbols are 0..n-1 with n <= 17 (the reference's MAX_N,
+ *    emitter/src/native/CMakeLists.txt:42); move bounds are 1..9
+ *    (MAX_DEPTH, :43).
+ *  - A move is a position triple (i, j, k), 0 <= i < j < k < n, in the beta
+ *    sequence current before the move; it swaps the blocks [i,j) and [j,k)
+ *    of beta and maps omega to omega o (beta[i] beta[j] beta[k])^-1
+ *    (emitter/src/native/backtracking/sorting_search.cu:228-230, 443-445).
+ *  - Every function returns a td16_status.  On failure a NUL-terminated
+ *    message is written to `msg` (when msg != NULL and msg_cap > 0); the JNI
+ *    shim turns every non-OK status into java.lang.RuntimeException(msg), as
+ *    jni_sorter.cu:15-22,111-116 does for every std::exception.
+ *  - Functions are thread-safe.  Calls on distinct slots run concurrently;
+ *    the reference also drives each slot from a single thread
+ *    (emitter/src/main/java/br/unb/cic/tdp/base/GPUSortingSearch.java:123-145).
+ */
+#ifndef TD16_ABI_H
+#define TD16_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TD16_MAX_N 17
+#define TD16_MAX_DEPTH 9
+
+typedef enum {
+    TD16_OK = 0,
+    TD16_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+    TD16_OUT_OF_MEMORY = 2,    /* GpuMemoryExhaustedException */
+    TD16_DEVICE_ERROR = 3,     /* CudaException */
+    TD16_NOT_INITIALIZED = 4,  /* search before initSlots */
+    TD16_QUEUE_FULL = 5        /* served loop: pending queue at capacity (try later) */
+} td16_status;
+
+/* Move sets.  REF is the reference filter (sorting_search.cu:375-380,
+ * 439-441, 607-610): no cut at an omega fixed point, not three distinct
+ * cycles, odd-cycle delta 0 or +2.  ALL takes every position triple. */
+enum { TD16_MOVES_REF = 0, TD16_MOVES_ALL = 1 };
+/* FIRST stops at the first witness (the reference's behaviour);
+ * COUNT_ALL never stops early and counts every node and witness. */
+enum { TD16_FIRST = 0, TD16_COUNT_ALL = 1 };
+
+/* One search request in the reference's form.
+ *   beta:  the cyclic sequence, beta[0..n-1] (a permutation of 0..n-1)
+ *   omega: one-line permutation, omega[x] = image of x
+ *   initial_even, min_rate, max_moves, full_sorting: as passed to
+ *     GPUSortingSearch.searchForSortingSeq (GPUSortingSearch.java:117)
+ *   moveset, mode: TD16_MOVES_*, TD16_*; the reference is REF + FIRST
+ *   iddfs: 1 = exact distance by iterative deepening on limits
+ *     max(ceil(b/3), odd-cycle bound), +1, ... up to max_moves (min_rate,
+ *     initial_even and full_sorting are then ignored: the witness condition
+ *     is "sorted"). */
+typedef struct {
+    uint8_t beta[TD16_MAX_N];
+    uint8_t omega[TD16_MAX_N];
+    uint8_t pad0[2];
+    int32_t n;
+    int32_t initial_even;
+    float min_rate;
+    int32_t max_moves;
+    int32_t full_sorting;
+    int32_t moveset;
+    int32_t mode;
+    int32_t iddfs;
+} td16_request;
+
+/* Result of one request.
+ *   status: td16_status of this request
+ *   found: 1 when a witness was found
+ *   depth: witness length (0 when none); in IDDFS mode the exact distance
+ *   moves: positions (i, j, k) of the witness, moves[0..depth-1]
+ *   expanded: nodes whose C(n,3) children were evaluated (root included)
+ *   solutions: witnesses met (COUNT_ALL: all of them)
+ *   split: 1 when the search was shared by several wavefronts (its FIRST
+ *     witness and counts may then differ from a sequential DFS's; found /
+ *     not-found never does) */
+typedef struct {
+    int32_t status;
+    int32_t found;
+    int32_t depth;
+    int32_t split;
+    int32_t moves[TD16_MAX_DEPTH][3];
+    uint64_t expanded;
+    uint64_t solutions;
+} td16_result;
+
+/* ---------------------------------------------------------------------- */
+/* Slots: the reference's per-slot lifecycle                              */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces Java_br_unb_cic_tdp_base_GPUSortingSearch_initSlots
+ * (jni_sorter.cu:24-47) -> initGpuSortSlots (backtracking/slot_manager.cu:
+ * 44-128) -> allocateGpuSortingSearchSlot (sorting_search.cu:755-793).
+ * Slot s runs on devices[s % ndevices].  Each slot gets its own stream and
+ * `queue_bytes_budget` bytes of HBM for its work queue.  Fails with
+ * TD16_INVALID_ARGUMENT on an empty, duplicate or out-of-range device list,
+ * total_slots <= 0 or budget <= 0, and with TD16_OUT_OF_MEMORY when
+ * budget x slots exceeds a device's free memory.  Re-initialising replaces
+ * the previous slots. */
+int td16_init_slots(int total_slots, const int32_t *devices, int ndevices,
+                    int64_t queue_bytes_budget, char *msg, int msg_cap);
+
+/* Releases every slot, stream and buffer (the reference never frees its
+ * slots; this exists for tests and orderly shutdown). */
+int td16_shutdown(char *msg, int msg_cap);
+
+/* Number of slots and the device of one slot (-1 when invalid).
+ * Replaces getGpuSortSlotDevice (slot_manager.cu:145-148). */
+int td16_slot_count(void);
+int td16_slot_device(int slot);
+
+/* ---------------------------------------------------------------------- */
+/* The drop-in search                                                      */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces Java_br_unb_cic_tdp_base_GPUSortingSearch_searchForSortingSeq
+ * (jni_sorter.cu:49-117) -> performGpuSortingSearch (sorting_search.cu:
+ * 811-972): one blocking search on `slot`, REF moves, FIRST mode.
+ * out_positions receives up to 9 position triples (row-major [9][3]);
+ * *out_len the witness length, 0 when there is none (including n < 3).
+ * Errors: TD16_INVALID_ARGUMENT when beta/omega are not permutations of
+ * 0..n-1, n > 17, max_moves outside 1..9 or the slot is invalid
+ * (sorting_search.cu:820-825, 140-145). */
+int td16_search(int slot, const uint8_t *beta, const uint8_t *omega, int n,
+                int initial_even, float min_rate, int max_moves,
+                int full_sorting, int32_t *out_positions, int32_t *out_len,
+                char *msg, int msg_cap);
+
+/* The same blocking one-instance-per-launch search with every option of
+ * td16_request and the counters of td16_result. */
+int td16_search_ex(int slot, const td16_request *req, td16_result *res,
+                   char *msg, int msg_cap);
+
+/* ---------------------------------------------------------------------- */
+/* Batched search: one persistent launch over many requests                */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+    int32_t device;          /* HIP device ordinal */
+    int32_t waves_per_cu;    /* 0 = occupancy maximum */
+    int32_t split_depth;     /* frames deeper than max_moves - split_depth are
+                                never donated; 0 = default (3) */
+    int32_t donate_max;      /* children donated per donation; 0 = default (64) */
+    int32_t use_graph;       /* 1 = launch through a captured hipGraph */
+    int32_t repeats;         /* run the same batch this many times (>= 1) */
+    int64_t queue_bytes;     /* HBM for the shared work queue; 0 = 256 MiB */
+} td16_batch_options;
+
+typedef struct {
+    double kernel_ms;        /* mean device time of one persistent launch */
+    double wall_ms;          /* mean host wall time of one batch (H2D+launch+D2H) */
+    uint64_t expanded;       /* sum over requests (last repeat) */
+    uint64_t solutions;
+    uint64_t donations;      /* work items pushed to the shared queue */
+    int32_t waves;           /* wavefronts in the persistent grid */
+    int32_t blocks;
+} td16_batch_stats;
+
+/* Runs `count` requests in one persistent, work-sharing launch per repeat.
+ * results[q] is request q's result (from the last repeat). */
+int td16_batch(const td16_request *reqs, int count, td16_result *results,
+               const td16_batch_options *opt, td16_batch_stats *stats,
+               char *msg, int msg_cap);
+
+/* ---------------------------------------------------------------------- */
+/* Served loop: continuous batching into a running persistent kernel       */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+    int32_t device;
+    int32_t slots;           /* requests admitted on the device at once (7..4096) */
+    int32_t queue_capacity;  /* requests waiting for a slot (the reference's
+                                tdp.gpuQueueCapacity, GPUSortingSearch.java:82-89) */
+    int32_t waves_per_cu;    /* 0 = occupancy maximum */
+    int64_t slot_bytes;      /* HBM budget per slot (tdp.gpuBudgetMB) */
+} td16_serve_options;
+
+typedef struct {
+    uint64_t submitted, completed, rejected;
+    double p50_us, p99_us, mean_us; /* submit -> result visible on the host */
+    int32_t in_flight, waiting;
+} td16_serve_stats;
+
+/* Starts the served loop on one device: a persistent kernel that admits
+ * requests as slots free up, shares their subtrees across wavefronts and
+ * reports each completion to the host. */
+int td16_serve_start(const td16_serve_options *opt, char *msg, int msg_cap);
+/* Queues a request; blocks while `queue_capacity` requests are waiting
+ * (GPUSortingSearch.java:216 blocks the same way) unless nonblocking, in
+ * which case TD16_QUEUE_FULL is returned.  *ticket identifies the request. */
+int td16_serve_submit(const td16_request *req, int nonblocking, uint64_t *ticket,
+                      char *msg, int msg_cap);
+/* Blocks until the ticket's result is ready and returns it. */
+int td16_serve_wait(uint64_t ticket, td16_result *res, char *msg, int msg_cap);
+/* Submit + wait. */
+int td16_serve_search(const td16_request *req, td16_result *res, char *msg, int msg_cap);
+int td16_serve_stats_get(td16_serve_stats *st, int reset_latency, char *msg, int msg_cap);
+/* Stops the kernel after the admitted requests finish. */
+int td16_serve_stop(char *msg, int msg_cap);
+
+/* ---------------------------------------------------------------------- */
+/* Caller-side replay and the certificate format                           */
+/* ---------------------------------------------------------------------- */
+
+/* Positions -> symbols by replaying the block swaps on a copy of beta
+ * (GPUSortingSearch.java:229-239,258-264). */
+int td16_positions_to_symbols(const uint8_t *beta, int n, const int32_t *positions,
+                              int len, int32_t *symbols, char *msg, int msg_cap);
+
+/* Replays symbol triples (CertificateValidator.java:14-50): length in
+ * 1..max_moves, each move an ordered triple of distinct symbols in the
+ * current beta, omega o beta preserved, omega sorted at the end.  Returns
+ * TD16_OK or TD16_INVALID_ARGUMENT with the reason. */
+int td16_validate(const uint8_t *beta, const uint8_t *omega, int n,
+                  const int32_t *symbols, int len, int max_moves,
+                  char *msg, int msg_cap);
+
+/* Certificate codec (CertificateCodec.java:22-118): the key is omega's
+ * one-line notation at 5 bits per image, MSB first; the value is 15 bits per
+ * move.  *out_len receives the byte count.  Decoders reject a wrong length
+ * or nonzero padding. */
+int td16_codec_key(const uint8_t *omega, int n, uint8_t *out, int cap, int32_t *out_len,
+                   char *msg, int msg_cap);
+int td16_codec_value(const int32_t *symbols, int len, uint8_t *out, int cap,
+                     int32_t *out_len, char *msg, int msg_cap);
+int td16_codec_decode_key(const uint8_t *in, int nbytes, int n, uint8_t *omega,
+                          char *msg, int msg_cap);
+int td16_codec_decode_value(const uint8_t *in, int nbytes, int32_t *symbols, int cap,
+                            int32_t *out_len, char *msg, int msg_cap);
+
+/* ---------------------------------------------------------------------- */
+/* Synthetic instances                                                     */
+/* ---------------------------------------------------------------------- */
+
+/* Linear permutations, nibble-packed (position p in bits 4p..4p+3, unused
+ * nibbles 0xF), from SplitMix64 with uniform draws by rejection.
+ *   uniform: Durstenfeld Fisher-Yates on 0..n-1, redrawn while identity
+ *   products: n ~ U{nmin..nmax}, k ~ U{kmin..kmax}, k uniform block
+ *     transpositions t(i,j,k), 0 <= i < j < k <= n, applied to the
+ *     identity; redrawn while identity.  ks receives k.
+ *   arrivals: unit-rate exponential gaps, -ln(1 - u), u = top 53 bits. */
+int td16_gen_uniform(uint64_t seed, int n, int count, uint64_t *perms);
+int td16_gen_products(uint64_t seed, int nmin, int nmax, int kmin, int kmax, int count,
+                      uint8_t *ns, uint8_t *ks, uint64_t *perms);
+int td16_gen_exponential(uint64_t seed, int count, double *gaps);
+
+/* A linear permutation of 0..n-1 as a request in the reference's form:
+ * beta = [pi_0 .. pi_{n-1}, n], omega(y) = pred(y) + 1 mod (n+1),
+ * initial_even = odd cycles of omega, min_rate = 2 / (k / (float)norm)
+ * (the emitter's convention, UnorientedCycles.java:34), max_moves = k. */
+int td16_linear_request(uint64_t perm, int n, int k, int moveset, int mode, int iddfs,
+                        td16_request *req);
+
+/* Library version string. */
+const char *td16_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

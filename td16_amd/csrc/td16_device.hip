@@ -1,0 +1,816 @@
+// Persistent work-sharing DFS for block-transposition witnesses on CDNA4
+// (gfx950, wave64).
+//
+// One wavefront is one DFS worker.  A node is expanded by the whole wave:
+//   1. set-up: lanes 0..N-1 own one sequence position each; the cycle
+//      structure of omega (cycle, length, distance to the cycle's smallest
+//      position) is found by 2 x 4 rounds of pointer jumping over
+//      ds_bpermute, no LDS tables and no serial walks;
+//   2. candidates: the C(m,3) position triples (m = breakpoints under the
+//      reference filter, all N positions otherwise) are evaluated 64 lanes at
+//      a time in lexicographic order; each lane gets the odd-cycle delta of
+//      its triple in O(1) from three bpermuted cycle records, then a ballot
+//      splits witnesses from expandable children.
+// The wave keeps one LDS frame per depth (sequence + expandable-children
+// bitmasks), so the DFS stack never spills.  Idle wavefronts take donated
+// subtrees from a lock-free ring in HBM, or the next request.
+//
+// Reference semantics followed (emitter/src/native/backtracking/
+// sorting_search.cu): move filter :375-380,439-441,607-610; delta cases
+// :240-332; witness and prune predicates :336-373 (host-tabulated); positions
+// returned exactly as :956-971 decodes them.
+#include <hip/hip_runtime.h>
+
+#include "td16_internal.h"
+
+namespace td16 {
+
+namespace {
+
+// --------------------------------------------------------------------------
+// Wave primitives
+// --------------------------------------------------------------------------
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int bperm(int srcLane, int v) {
+  return __builtin_amdgcn_ds_bpermute(srcLane << 2, v);
+}
+
+__device__ __forceinline__ int fperm(int dstLane, int v) {
+  return __builtin_amdgcn_ds_permute(dstLane << 2, v);
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Orders this wave's LDS writes before later reads by other lanes of the
+// same wave (LDS requests of one wave complete in order; this stops the
+// compiler from moving accesses across the point).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int rank_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+}
+
+template <class T>
+__device__ __forceinline__ T ld_rlx(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T ld_acq(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_rel(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_rlx(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T add_rlx(T *p, T v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T add_ar(T *p, T v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ bool cas(T *p, T expected, T desired) {
+  return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_ACQ_REL,
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ T ld_sys_acq(T *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <class T>
+__device__ __forceinline__ void st_sys_rel(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Lane 0 runs f, every lane gets its int result.
+template <class F>
+__device__ __forceinline__ int lane0(F f) {
+  int r = 0;
+  if (lane_id() == 0) r = f();
+  return uni(r);
+}
+
+// Block transposition t(i,j,k) on a nibble-packed sequence: blocks [i,j) and
+// [j,k) swap places (reference: emitter/src/native/backtracking/
+// transposition.cuh:5-11, E/base/GPUSortingSearch.java:258-264).
+__device__ __forceinline__ uint64_t apply_move(uint64_t pi, int i, int j, int k) {
+  const uint32_t lo = 4u * i, mid = 4u * j, hi = 4u * k;
+  const uint64_t mLo = (1ULL << lo) - 1ULL;
+  const uint64_t mMid = (1ULL << mid) - 1ULL;
+  const uint64_t mHi = (hi >= 64u) ? ~0ULL : ((1ULL << hi) - 1ULL);
+  const uint64_t A = pi & (mMid ^ mLo);
+  const uint64_t B = pi & (mHi ^ mMid);
+  const uint64_t rest = pi & ~(mHi ^ mLo);
+  return rest | (B >> (mid - lo)) | (A << (hi - mid));
+}
+
+__device__ __forceinline__ int choose3(int m) { return m < 3 ? 0 : m * (m - 1) * (m - 2) / 6; }
+__device__ __forceinline__ int choose4(int m) {
+  return m < 4 ? 0 : m * (m - 1) * (m - 2) * (m - 3) / 24;
+}
+
+// --------------------------------------------------------------------------
+// Per-wave LDS state
+// --------------------------------------------------------------------------
+
+struct Frame {
+  uint64_t pi;
+  uint64_t mask[kMaxChunks];  // expandable children, bit t%64 of chunk t/64
+  uint8_t m;                  // enumerated elements (breakpoints or all N)
+  uint8_t full;               // 1: element q is position q
+  uint8_t cursor;             // first chunk that may still hold bits
+  uint8_t nchunks;
+  uint8_t pos[20];            // element -> position
+};
+
+struct WaveLds {
+  Frame frames[kMaxDepth];
+  uint32_t satTab[kMaxDepth + 1];
+  uint32_t expTab[kMaxDepth + 1];
+  uint16_t path[kMaxDepth + 1];  // path[d]: move applied at depth d
+  uint16_t pad[6];
+};
+
+struct BlockLds {
+  uint16_t lex[kLexEntries];
+  uint16_t lexPad[4];
+  WaveLds wave[kWavesPerBlock];
+};
+
+// Scalar (wave-uniform) description of the item being searched.
+struct Ctx {
+  int N, n;
+  int flags;
+  int limit;       // effective depth limit
+  uint32_t job;
+  int Rreg;        // lane y holds R[y]
+  int depth0;
+  bool shared;     // other wavefronts may work on this request
+  uint64_t expanded;
+  uint64_t solutions;
+  int sinceCheck;
+};
+
+enum { kExpandOk = 0, kExpandFound = 1 };
+
+// Packs the witness into the job state when this wave is the first finder.
+__device__ void record_witness(const Params &P, Ctx &c, WaveLds &w, int depth, int lastMove) {
+  JobState *st = &P.states[c.job];
+  const int won = lane0([&] { return cas(&st->found, 0, 1) ? 1 : 0; });
+  if (!won) return;
+  const int lane = lane_id();
+  if (lane < depth) {
+    const uint16_t mv = (lane == depth - 1) ? (uint16_t)lastMove : w.path[lane];
+    st->moves[lane] = mv;
+  }
+  if (lane == 0) st->depth = depth;
+}
+
+// Expands the node `pi` at depth d into frame d.  Returns kExpandFound when
+// a witness was recorded (FIRST mode) and the item must stop.
+__device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, int d,
+                      uint64_t pi) {
+  const int lane = lane_id();
+  const int N = c.N, n = c.n;
+  Frame &F = w.frames[d];
+  c.expanded++;
+
+  // ---- set-up: lane p owns position p ------------------------------------
+  const int p = lane;
+  const bool inN = p < N;
+  const int sh = (4 * p) & 63;
+  const int shPrev = (4 * (p - 1)) & 63;
+  const int sPrev = (p == 0) ? n : (int)((pi >> shPrev) & 15u);
+  const int om = bperm(sPrev & 31, c.Rreg);  // omega(s_p) = R(s_{p-1})
+  int nx;
+  if (om == n) {
+    nx = n;
+  } else {
+    const uint64_t x = pi ^ ((uint64_t)(om & 15) * 0x1111111111111111ULL);
+    const uint64_t z = ((x - 0x1111111111111111ULL) & ~x & 0x8888888888888888ULL) |
+                       0x8000000000000000ULL;
+    nx = __builtin_ctzll(z) >> 2;
+  }
+  nx = inN ? nx : p;  // lanes past N: harmless fixed points
+  (void)sh;
+
+  // cycle members within distance 16: 4 doubling rounds
+  int J = nx & 31;
+  uint32_t M = (1u << (p & 31)) | (1u << J);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int got = bperm(J, (int)(((uint32_t)J) | (M << 5)));
+    M |= ((uint32_t)got) >> 5;
+    J = got & 31;
+  }
+  const int leader = __builtin_ctz(M);
+  const int L = __builtin_popcount(M);
+  // forward distance to the leader: 4 doubling rounds with the leader absorbing
+  const bool isLeader = (p == leader);
+  int J2 = isLeader ? p : (nx & 31);
+  int D = isLeader ? 0 : 1;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int got = bperm(J2, J2 | (D << 5));
+    D += got >> 5;
+    J2 = got & 31;
+  }
+  const int ec = __builtin_popcountll(ballot(inN && isLeader && (L & 1)));
+  const uint64_t bpMask = ballot(inN && nx != p);
+  const int info = D | (L << 8) | (leader << 16);
+
+  // ---- enumeration set --------------------------------------------------
+  const int cd = d + 1;
+  const uint32_t satMask = w.satTab[cd];
+  const uint32_t expMask = w.expTab[cd];
+  const bool allMoves = (c.flags & kFlagAllMoves) != 0;
+  bool full = false;
+  if (allMoves) {
+    // triples through an omega fixed point change the odd-cycle count by
+    // 0 or -2; enumerate them only when such a child can still matter
+    uint32_t low = 1u << ec;
+    if (ec >= 2) low |= 1u << (ec - 2);
+    full = ((satMask | expMask) & low) != 0;
+  }
+  int m, einfo;
+  if (full) {
+    m = N;
+    einfo = info;
+  } else {
+    m = __builtin_popcountll(bpMask);
+    const bool isBp = inN && nx != p;
+    const int r = rank_below(bpMask);
+    einfo = fperm(isBp ? r : 63, info);
+    if (isBp) F.pos[r] = (uint8_t)p;
+  }
+  const int ncand = choose3(m);
+  const int nchunks = (ncand + 63) >> 6;
+  const int off = choose4(m);
+  const int mode = c.flags & kFlagCountAll;
+
+  // ---- candidates ------------------------------------------------------
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int t = (ch << 6) + lane;
+    const bool valid = t < ncand;
+    const int e = valid ? (int)lex[off + t] : 0;
+    const int A = bperm(e & 31, einfo);
+    const int B = bperm((e >> 5) & 31, einfo);
+    const int C = bperm((e >> 10) & 31, einfo);
+    const int Da = A & 31, La = (A >> 8) & 31, Ca = A >> 16;
+    const int Db = B & 31, Lb = (B >> 8) & 31, Cb = B >> 16;
+    const int Dc = C & 31, Lc = (C >> 8) & 31, Cc = C >> 16;
+    const bool sab = Ca == Cb, sbc = Cb == Cc, sac = Ca == Cc;
+    int delta;
+    bool acc;
+    if (sab && sbc) {
+      // one cycle: order a->b->c splits it in three, a->c->b keeps it
+      int dab = Da - Db; dab += dab < 0 ? La : 0;
+      int dbc = Db - Dc; dbc += dbc < 0 ? La : 0;
+      int dca = Dc - Da; dca += dca < 0 ? La : 0;
+      const int odd = (dab & 1) + (dbc & 1) + (dca & 1);
+      delta = (dab + dbc + dca == La) ? odd - (La & 1) : 0;
+      acc = true;
+    } else if (sab || sbc || sac) {
+      // x -> y on a shared cycle of length L1, the third symbol alone on L2;
+      // the x..y segment closes on itself, the rest joins the lone cycle
+      const int L1 = sab ? La : (sbc ? Lb : La);
+      const int L2 = sab ? Lc : (sbc ? La : Lb);
+      const int Dx = sab ? Da : (sbc ? Db : Dc);
+      const int Dy = sab ? Db : (sbc ? Dc : Da);
+      int dd = Dx - Dy; dd += dd < 0 ? L1 : 0;
+      const int l1 = L1 & 1, l2 = L2 & 1;
+      delta = (l1 == l2) ? 2 * (dd & 1) - 2 * l1 : 0;
+      acc = true;
+    } else {
+      // three cycles merge; the reference filter skips these
+      delta = ((La ^ Lb ^ Lc) & 1) - (La & 1) - (Lb & 1) - (Lc & 1);
+      acc = allMoves;
+    }
+    if (!allMoves) acc = acc && delta >= 0;
+    const int cec = (ec + delta) & 31;
+    const bool isSat = valid && acc && ((satMask >> cec) & 1u);
+    const bool isExp = valid && acc && !((satMask >> cec) & 1u) && ((expMask >> cec) & 1u);
+    const uint64_t satb = ballot(isSat);
+    const uint64_t expb = ballot(isExp);
+    if (satb != 0) {
+      if (mode != 0) {
+        c.solutions += (uint64_t)__builtin_popcountll(satb);
+      } else {
+        wave_sync();
+        const int wl = __builtin_ctzll(satb);
+        const int we = bperm(wl, e);
+        const int wi = we & 31, wj = (we >> 5) & 31, wk = (we >> 10) & 31;
+        int pi_ = wi, pj = wj, pk = wk;
+        if (!full) {
+          pi_ = F.pos[wi];
+          pj = F.pos[wj];
+          pk = F.pos[wk];
+        }
+        c.solutions += 1;
+        record_witness(P, c, w, cd, pi_ | (pj << 5) | (pk << 10));
+        return kExpandFound;
+      }
+    }
+    if (lane == 0) F.mask[ch] = expb;
+  }
+  if (lane == 0) {
+    F.pi = pi;
+    F.m = (uint8_t)m;
+    F.full = full ? 1 : 0;
+    F.cursor = 0;
+    F.nchunks = (uint8_t)nchunks;
+  }
+  wave_sync();
+  return kExpandOk;
+}
+
+// Decodes bit `bit` of chunk `ch` of frame F into a packed move; every lane
+// decodes its own bit (no cross-lane step).
+__device__ __forceinline__ int frame_move_lane(const Frame &F, const uint16_t *lex, int ch,
+                                               int bit) {
+  const int m = F.m;
+  const int e = lex[choose4(m) + (ch << 6) + bit];
+  int i = e & 31, j = (e >> 5) & 31, k = (e >> 10) & 31;
+  if (!F.full) {
+    i = F.pos[i];
+    j = F.pos[j];
+    k = F.pos[k];
+  }
+  return i | (j << 5) | (k << 10);
+}
+
+// Wave-uniform version: `ch` and `bit` are the same on every lane.
+__device__ __forceinline__ int frame_move(const Frame &F, const uint16_t *lex, int ch, int bit) {
+  return uni(frame_move_lane(F, lex, ch, bit));
+}
+
+// ---------------------------------------------------------------------------
+// Shared queue
+// ---------------------------------------------------------------------------
+
+// Tries to donate the still-unvisited children of the shallowest frame that
+// has any (within the split depth).  Returns the number donated.
+__device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, int d) {
+  const int lane = lane_id();
+  const int maxFd = c.limit - P.splitSlack;
+  int fd = -1, ch = -1;
+  for (int q = c.depth0; q <= d && q <= maxFd; ++q) {
+    Frame &F = w.frames[q];
+    const int nch = F.nchunks;
+    for (int x = F.cursor; x < nch; ++x) {
+      if (F.mask[x] != 0) {
+        fd = q;
+        ch = x;
+        break;
+      }
+    }
+    if (fd >= 0) break;
+  }
+  fd = uni(fd);
+  ch = uni(ch);
+  if (fd < 0) return 0;
+  Frame &F = w.frames[fd];
+  uint64_t bits = uni64(F.mask[ch]);
+  // keep at most donateMax of them (lowest bits go)
+  int cnt = __builtin_popcountll(bits);
+  if (cnt > P.donateMax) {
+    uint64_t keep = bits;
+    for (int q = 0; q < P.donateMax; ++q) keep &= keep - 1;
+    bits ^= keep;
+    cnt = P.donateMax;
+  }
+  // reserve cnt consecutive ring slots
+  const uint64_t cap = P.qcap;
+  int ok = 0;
+  uint64_t t0 = 0;
+  for (int tries = 0; tries < 4 && !ok; ++tries) {
+    t0 = uni64(lane == 0 ? ld_rlx(&P.queue->tail) : 0);
+    bool busy = false;
+    if (lane < cnt) {
+      const uint64_t pos = t0 + (uint64_t)lane;
+      busy = ld_acq(&P.seq[pos % cap]) != pos;
+    }
+    if (ballot(busy) != 0) return 0;  // ring full (or racing): keep working locally
+    ok = lane0([&] { return cas(&P.queue->tail, t0, t0 + (uint64_t)cnt) ? 1 : 0; });
+  }
+  if (!ok) return 0;
+  // account before publishing so the request cannot complete underneath
+  if (lane == 0) {
+    add_ar(&P.states[c.job].pending, cnt);
+    st_rlx(&P.states[c.job].split, 1);
+    add_rlx(&P.ctrl->donations, (uint64_t)cnt);
+  }
+  const bool mine = (bits >> lane) & 1ULL;
+  if (mine) {
+    const int slot = rank_below(bits);
+    const int mv = frame_move_lane(F, lex, ch, lane);
+    Item it;
+    it.pi = apply_move(F.pi, mv & 31, (mv >> 5) & 31, (mv >> 10) & 31);
+    it.job = c.job;
+    it.depth = (uint32_t)(fd + 1);
+#pragma unroll
+    for (int q = 0; q < 10; ++q)
+      it.moves[q] = (q < fd) ? w.path[q] : (q == fd ? (uint16_t)mv : (uint16_t)0);
+    it.pad[0] = it.pad[1] = it.pad[2] = 0;
+    const uint64_t pos = t0 + (uint64_t)slot;
+    P.items[pos % cap] = it;
+    __threadfence();
+    st_rel(&P.seq[pos % cap], pos + 1);
+  }
+  if (lane == 0) F.mask[ch] &= ~bits;
+  wave_sync();
+  c.shared = true;
+  return cnt;
+}
+
+// Pops one item; returns 1 and fills `it` when there was one.
+__device__ int pop_item(const Params &P, Item &it, WaveLds &w) {
+  const int lane = lane_id();
+  const uint64_t cap = P.qcap;
+  int got = 0;
+  uint64_t h = 0;
+  if (lane == 0) {
+    for (int tries = 0; tries < 8; ++tries) {
+      h = ld_rlx(&P.queue->head);
+      const uint64_t s = ld_acq(&P.seq[h % cap]);
+      const int64_t dif = (int64_t)(s - (h + 1));
+      if (dif == 0) {
+        if (cas(&P.queue->head, h, h + 1)) {
+          got = 1;
+          break;
+        }
+      } else if (dif < 0) {
+        break;  // empty
+      }
+    }
+  }
+  got = uni(got);
+  if (!got) return 0;
+  h = uni64(h);
+  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  it = P.items[h % cap];
+  if (lane < 10) w.path[lane] = P.items[h % cap].moves[lane];
+  __threadfence();
+  if (lane == 0) st_rel(&P.seq[h % cap], h + cap);
+  wave_sync();
+  return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Requests
+// ---------------------------------------------------------------------------
+
+// Loads the request's parameters into the wave context and LDS tables.
+__device__ void load_job(const Params &P, Ctx &c, WaveLds &w, uint32_t j) {
+  const int lane = lane_id();
+  const Job *job = &P.jobs[j];
+  c.job = j;
+  c.N = uni(job->N);
+  c.n = c.N - 1;
+  c.flags = uni(job->flags);
+  const int limit = uni(ld_rlx(&P.states[j].limit));
+  c.limit = limit;
+  c.Rreg = lane < 20 ? (int)job->R[lane < 20 ? lane : 0] : 0;
+  if (lane <= kMaxDepth) {
+    uint32_t s, r;
+    if (c.flags & kFlagIddfs) {
+      s = lane >= 1 ? (1u << c.N) : 0u;
+      const int left = limit - lane;
+      const int lo = c.N - 2 * left;
+      r = left < 0 ? 0u : (lo <= 0 ? ((2u << c.N) - 1u) : (((2u << c.N) - 1u) & ~((1u << lo) - 1u)));
+    } else {
+      s = job->sat[lane];
+      r = job->reach[lane];
+    }
+    w.satTab[lane] = s;
+    w.expTab[lane] = (lane < limit) ? r : 0u;
+  }
+  wave_sync();
+}
+
+// Runs one item (a node to expand and its whole subtree) to completion.
+// Returns true when this wave released the request's last pending item.
+__device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
+                         uint64_t pi, int depth0) {
+  const int lane = lane_id();
+  c.depth0 = depth0;
+  c.expanded = 0;
+  c.solutions = 0;
+  c.sinceCheck = 0;
+  const bool countAll = (c.flags & kFlagCountAll) != 0;
+  JobState *st = &P.states[c.job];
+
+  bool stop = false;
+  if (c.shared && !countAll) stop = lane0([&] { return ld_rlx(&st->found); }) != 0;
+  if (!stop && expand(P, c, w, lex, depth0, pi) == kExpandFound) stop = true;
+  int d = depth0;
+  while (!stop) {
+    Frame &F = w.frames[d];
+    // next expandable child of the top frame
+    int ch = F.cursor;
+    const int nch = F.nchunks;
+    uint64_t bits = 0;
+    while (ch < nch) {
+      bits = uni64(F.mask[ch]);
+      if (bits) break;
+      ++ch;
+    }
+    if (ch >= nch) {
+      if (d == depth0) break;
+      --d;
+      continue;
+    }
+    const int bit = __builtin_ctzll(bits);
+    const int mv = frame_move(F, lex, ch, bit);
+    const uint64_t child = apply_move(uni64(F.pi), mv & 31, (mv >> 5) & 31, (mv >> 10) & 31);
+    if (lane == 0) {
+      F.mask[ch] = bits & (bits - 1);
+      F.cursor = (uint8_t)ch;
+      w.path[d] = (uint16_t)mv;
+    }
+    wave_sync();
+    // sharing and cancellation checks
+    if (++c.sinceCheck >= 16) {
+      c.sinceCheck = 0;
+      const int idle = lane0([&] { return ld_rlx(&P.ctrl->idle); });
+      if (idle > 0) donate(P, c, w, lex, d);
+      if (c.shared && !countAll) {
+        if (lane0([&] { return ld_rlx(&st->found); }) != 0) break;
+      }
+    }
+    ++d;
+    if (expand(P, c, w, lex, d, child) == kExpandFound) break;
+  }
+  // flush counters, then release this item
+  int last = 0;
+  if (lane == 0) {
+    if (c.expanded) add_rlx(&st->expanded, c.expanded);
+    if (c.solutions) add_rlx(&st->solutions, c.solutions);
+    last = add_ar(&st->pending, -1) == 1 ? 1 : 0;
+  }
+  return uni(last) != 0;
+}
+
+__device__ void publish_result(const Params &P, uint32_t j) {
+  const int lane = lane_id();
+  JobState *st = &P.states[j];
+  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  if (!P.served) {
+    if (lane == 0) {
+      DevResult r;
+      r.found = st->found;
+      r.depth = st->depth;
+      r.split = st->split;
+      r.limit = st->limit;
+      r.expanded = st->expanded;
+      r.solutions = st->solutions;
+      for (int q = 0; q < 10; ++q) r.moves[q] = st->moves[q];
+      r.tag = P.jobs[j].tag;
+      r.seq = 0;
+      r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0;
+      P.results[j] = r;
+      __threadfence();
+      add_rlx(&P.ctrl->jobsDone, 1u);
+    }
+  } else {
+    if (lane == 0) {
+      const uint64_t cseq = add_rlx(&P.ctrl->completions, (uint64_t)1);
+      const uint32_t ringIdx = j;  // job slot
+      DevResult r;
+      r.found = st->found;
+      r.depth = st->depth;
+      r.split = st->split;
+      r.limit = st->limit;
+      r.expanded = st->expanded;
+      r.solutions = st->solutions;
+      for (int q = 0; q < 10; ++q) r.moves[q] = st->moves[q];
+      r.tag = P.jobs[j].tag;
+      r.seq = 0;
+      r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0;
+      P.hostResults[ringIdx] = r;
+      const uint32_t cslot = (uint32_t)(cseq & (uint64_t)(P.ringCap - 1));
+      P.hostCompletionIdx[2 * cslot] = ringIdx;
+      __threadfence_system();
+      st_sys_rel(&P.hostCompletionIdx[2 * cslot + 1], (uint32_t)(cseq + 1));
+      add_rlx(&P.ctrl->jobsDone, 1u);
+    }
+  }
+}
+
+// Starts request j (root item) in this wave.
+__device__ void start_job(const Params &P, uint32_t j) {
+  const int lane = lane_id();
+  if (lane == 0) {
+    JobState *st = &P.states[j];
+    const Job *job = &P.jobs[j];
+    st->pending = 1;
+    st->found = 0;
+    st->limit = (job->flags & kFlagIddfs) ? job->firstLimit : job->maxDepth;
+    st->split = 0;
+    st->expanded = 0;
+    st->solutions = 0;
+    st->depth = 0;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+}
+
+// Served loop: copies a newly submitted request into the device job table.
+__device__ int take_submitted(const Params &P, uint32_t &jOut) {
+  const int lane = lane_id();
+  int got = 0;
+  uint64_t r = 0;
+  if (lane == 0) {
+    uint64_t avail = ld_rlx(&P.ctrl->submittedMirror);
+    r = ld_rlx(&P.ctrl->consumed);
+    if (r >= avail) {
+      // refresh the mirror from host memory at most every ~2 us
+      const uint64_t now = __builtin_readcyclecounter();
+      const uint64_t last = ld_rlx(&P.ctrl->lastPoll);
+      if (now - last > 4000ULL && cas(&P.ctrl->lastPoll, last, now)) {
+        avail = ld_sys_acq(&P.host->submitted);
+        uint64_t cur = ld_rlx(&P.ctrl->submittedMirror);
+        while (cur < avail && !cas(&P.ctrl->submittedMirror, cur, avail))
+          cur = ld_rlx(&P.ctrl->submittedMirror);
+      }
+      avail = ld_rlx(&P.ctrl->submittedMirror);
+    }
+    while (r < avail) {
+      if (cas(&P.ctrl->consumed, r, r + 1)) {
+        got = 1;
+        break;
+      }
+      r = ld_rlx(&P.ctrl->consumed);
+    }
+  }
+  got = uni(got);
+  if (!got) return 0;
+  r = uni64(r);
+  // ring entries are transport only: the record's tag names its job slot
+  const uint32_t e = (uint32_t)(r & (uint64_t)(P.ringCap - 1));
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(&P.hostJobs[e]);
+  const uint32_t j = (uint32_t)uni((int)__hip_atomic_load(&P.hostJobs[e].tag, __ATOMIC_ACQUIRE,
+                                                         __HIP_MEMORY_SCOPE_SYSTEM));
+  if (lane < 8) {
+    uint32_t *dst = reinterpret_cast<uint32_t *>(const_cast<Job *>(&P.jobs[j]));
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      dst[4 * lane + q] = __hip_atomic_load(src + 4 * lane + q, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+  jOut = j;
+  return 1;
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(kBlockThreads)
+td16_search_kernel(Params P) {
+  __shared__ BlockLds lds;
+  for (int q = threadIdx.x; q < kLexEntries; q += blockDim.x) lds.lex[q] = P.lex[q];
+  __syncthreads();
+
+  const int wid = threadIdx.x >> 6;
+  WaveLds &w = lds.wave[wid];
+  const uint16_t *lex = lds.lex;
+  const int lane = lane_id();
+
+  Ctx c;
+  c.shared = false;
+  bool idle = false;
+  int backoff = 1;
+  for (;;) {
+    Item it;
+    uint32_t j = 0;
+    bool haveItem = pop_item(P, it, w) != 0;
+    bool haveJob = false;
+    if (!haveItem) {
+      if (!P.served) {
+        const int jj = lane0([&] {
+          const uint32_t nj = ld_rlx(&P.ctrl->nextJob);
+          if (nj >= P.ctrl->numJobs) return -1;
+          const uint32_t got = add_rlx(&P.ctrl->nextJob, 1u);
+          return got < P.ctrl->numJobs ? (int)got : -1;
+        });
+        if (jj >= 0) {
+          j = (uint32_t)jj;
+          haveJob = true;
+        }
+      } else {
+        haveJob = take_submitted(P, j) != 0;
+      }
+    }
+    if (!haveItem && !haveJob) {
+      if (!idle) {
+        if (lane == 0) add_rlx(&P.ctrl->idle, 1);
+        idle = true;
+      }
+      int done = 0;
+      if (!P.served) {
+        done = lane0([&] { return ld_rlx(&P.ctrl->jobsDone) >= P.ctrl->numJobs ? 1 : 0; });
+      } else {
+        done = lane0([&] {
+          if (ld_sys_acq(&P.host->stop) == 0) return 0;
+          const uint64_t cons = ld_rlx(&P.ctrl->consumed);
+          const uint64_t sub = ld_sys_acq(&P.host->submitted);
+          return (cons >= sub && ld_rlx(&P.ctrl->jobsDone) >= (uint32_t)cons) ? 1 : 0;
+        });
+      }
+      if (done) break;
+      for (int q = 0; q < backoff; ++q) __builtin_amdgcn_s_sleep(2);
+      backoff = backoff < 16 ? backoff * 2 : 16;
+      continue;
+    }
+    if (idle) {
+      if (lane == 0) add_rlx(&P.ctrl->idle, -1);
+      idle = false;
+    }
+    backoff = 1;
+
+    uint64_t pi;
+    int depth0;
+    if (haveJob) {
+      start_job(P, j);
+      pi = uni64(P.jobs[j].pi0);
+      depth0 = 0;
+      c.shared = false;
+    } else {
+      j = (uint32_t)uni((int)it.job);
+      pi = uni64(it.pi);
+      depth0 = uni((int)it.depth);
+      c.shared = true;
+    }
+    load_job(P, c, w, j);
+    for (;;) {
+      if (!run_item(P, c, w, lex, pi, depth0)) break;
+      // last item of request j finished: next IDDFS limit or publish
+      JobState *st = &P.states[j];
+      const int again = lane0([&] {
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        const Job *job = &P.jobs[j];
+        if ((job->flags & kFlagIddfs) && st->found == 0 && st->limit < job->maxDepth) {
+          st->limit = st->limit + 1;
+          st->pending = 1;
+          __threadfence();
+          return 1;
+        }
+        return 0;
+      });
+      if (!again) {
+        publish_result(P, j);
+        break;
+      }
+      load_job(P, c, w, j);
+      pi = uni64(P.jobs[j].pi0);
+      depth0 = 0;
+      c.shared = false;
+    }
+  }
+  if (idle && lane == 0) add_rlx(&P.ctrl->idle, -1);
+}
+
+__global__ void td16_queue_init_kernel(uint64_t *seq, uint64_t cap) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cap) seq[i] = i;
+}
+
+int launch_search(const Params &p, int blocks, void *stream) {
+  hipLaunchKernelGGL(td16_search_kernel, dim3(blocks), dim3(kBlockThreads), 0,
+                     (hipStream_t)stream, p);
+  return (int)hipGetLastError();
+}
+
+int launch_queue_init(uint64_t *seq, uint64_t cap, void *stream) {
+  const uint64_t blocks = (cap + 255) / 256;
+  hipLaunchKernelGGL(td16_queue_init_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, seq, cap);
+  return (int)hipGetLastError();
+}
+
+int search_kernel_occupancy(int *blocksPerCU) {
+  return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, td16_search_kernel,
+                                                           kBlockThreads, 0);
+}
+
+}  // namespace td16

@@ -1,0 +1,155 @@
+// Internal layout shared by the HIP kernels and the host runtime of libtd16.
+//
+// Search state
+//   A search is carried in "label" form: the beta sequence of the request is
+//   relabelled so that label(beta0[p]) = p.  The start sequence is then the
+//   identity 0..n-1 (n = N-1) with the sentinel label n fixed at position n
+//   (moves t(i,j,k) have k <= n, so position n never moves).  Positions
+//   0..n-1 are nibble-packed into one uint64 (position p in bits 4p..4p+3;
+//   n <= 16).  The request's target rho = omega0 o beta0 becomes a map R on
+//   labels, and omega at any node is omega(y) = R(pred(y)), pred being the
+//   cyclic predecessor in the sequence.  The reference keeps beta's function
+//   form implicit the same way through rho^-1
+//   (emitter/src/native/backtracking/sorting_search.cu:542-561, 899-913).
+#pragma once
+
+#include <cstdint>
+
+namespace td16 {
+
+constexpr int kMaxN = 17;
+constexpr int kMaxDepth = 9;
+constexpr int kMaxChunks = 11;      // ceil(C(17,3) / 64)
+constexpr int kLexEntries = 3060;   // sum over m = 3..17 of C(m,3) = C(18,4)
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlockThreads = 64 * kWavesPerBlock;
+
+enum JobFlags : uint8_t {
+  kFlagAllMoves = 1,   // every position triple (else the reference filter)
+  kFlagCountAll = 2,   // no early exit
+  kFlagIddfs = 4,      // iterative deepening on the depth limit
+};
+
+// One request as the kernel reads it (host-written, 128 B).
+struct alignas(16) Job {
+  uint64_t pi0;          // start sequence (labels), nibble-packed
+  uint8_t R[20];         // omega(y) = R[pred(y)]; R[0..N-1] used
+  uint8_t N;             // symbols = n + 1 (3..17)
+  uint8_t maxDepth;      // move bound (IDDFS: last limit tried)
+  uint8_t flags;         // JobFlags
+  uint8_t firstLimit;    // IDDFS: first limit
+  uint32_t sat[10];      // bit e of sat[d]: a depth-d child with e odd cycles is a witness
+  uint32_t reach[10];    // bit e of reach[d]: such a child may still lead to one
+  uint32_t pad[3];
+  uint32_t tag;          // served loop: low bits of the ticket
+};
+static_assert(sizeof(Job) == 128, "Job layout");
+
+// Mutable per-request state on the device (64 B).
+struct alignas(64) JobState {
+  int32_t pending;       // work items of this request not yet finished
+  int32_t found;         // 0/1, set once by the wavefront that records the witness
+  int32_t limit;         // current depth limit
+  int32_t split;         // 1 once a subtree was donated
+  uint64_t expanded;
+  uint64_t solutions;
+  int32_t depth;         // witness length
+  uint16_t moves[10];    // witness, packed i | j<<5 | k<<10
+  uint8_t pad[4];
+};
+static_assert(sizeof(JobState) == 64, "JobState layout");
+
+// What the host reads back per request (64 B).
+struct alignas(16) DevResult {
+  int32_t found;
+  int32_t depth;
+  int32_t split;
+  int32_t limit;
+  uint64_t expanded;
+  uint64_t solutions;
+  uint16_t moves[10];
+  uint32_t tag;
+  uint32_t seq;          // served loop: completion sequence + 1 once written
+  uint8_t pad[4];
+};
+static_assert(sizeof(DevResult) == 64, "DevResult layout");
+
+// A donated subtree: a node still to be expanded (48 B).
+struct alignas(16) Item {
+  uint64_t pi;
+  uint32_t job;
+  uint32_t depth;        // moves already applied
+  uint16_t moves[10];    // the moves that led here, packed
+  uint32_t pad[3];
+};
+static_assert(sizeof(Item) == 48, "Item layout");
+
+// Shared work queue: a bounded lock-free ring (per-slot sequence numbers,
+// D. Vyukov's bounded MPMC scheme).  head/tail only grow and persist across
+// launches; every launch ends with the queue empty.
+struct alignas(128) QueueState {
+  uint64_t head;
+  uint8_t pad0[120];
+  uint64_t tail;
+  uint8_t pad1[120];
+};
+
+// Per-launch control block (reset by the host before each launch).
+struct alignas(128) Control {
+  uint32_t nextJob;      // batched feed cursor
+  uint32_t numJobs;
+  uint32_t pad0[30];
+  uint32_t jobsDone;
+  uint32_t pad1[31];
+  int32_t idle;          // wavefronts currently without work
+  uint32_t pad2[31];
+  uint64_t donations;
+  uint64_t pad3[15];
+  // served loop
+  uint64_t consumed;     // submit-ring entries taken by the device
+  uint64_t completions;  // completion-ring entries written
+  uint64_t lastPoll;     // device clock of the last host-ring poll
+  uint64_t submittedMirror;
+  uint64_t pad4[12];
+};
+
+// Host-visible rings of the served loop (fine-grained pinned host memory).
+struct alignas(128) ServeHost {
+  uint64_t submitted;    // entries published by the host
+  uint64_t pad0[15];
+  uint32_t stop;         // host asks the kernel to exit once idle
+  uint32_t pad1[31];
+};
+
+struct Params {
+  const Job *jobs;
+  JobState *states;
+  DevResult *results;
+  Control *ctrl;
+  QueueState *queue;
+  Item *items;
+  uint64_t *seq;
+  uint64_t qcap;
+  const uint16_t *lex;   // lexicographic triple tables, kLexEntries entries
+  int32_t splitSlack;    // donate only frames with depth <= limit - splitSlack
+  int32_t donateMax;
+  int32_t served;        // 0 batched, 1 served loop
+  uint32_t ringCap;      // served: power of two
+  ServeHost *host;       // served: device view of the host block
+  const Job *hostJobs;   // served: device view of the host submit ring
+  DevResult *hostResults;// served: device view of the host completion ring
+  uint32_t *hostCompletionIdx; // served: ring index per completion sequence
+};
+
+// Launch wrappers (td16_device.hip).
+int launch_search(const Params &p, int blocks, void *stream);
+int launch_queue_init(uint64_t *seq, uint64_t cap, void *stream);
+int search_kernel_occupancy(int *blocksPerCU);
+
+// Host helpers (td16_host.cpp).
+void build_lex_table(uint16_t *out);   // kLexEntries entries
+int rate_tables(int N, int initialEven, float minRate, int maxDepth, int fullSorting,
+                uint32_t sat[10], uint32_t reach[10]);
+void unpack_moves(const uint16_t *packed, int depth, int32_t (*moves)[3]);
+
+}  // namespace td16

@@ -1,0 +1,272 @@
+"""Pythonic wrappers over the C ABI of libtd16.so.
+
+Requests use the reference's form: ``beta`` is the cyclic sequence and
+``omega`` the one-line algebraic permutation, both permutations of 0..n-1
+(reference: emitter/src/main/java/br/unb/cic/tdp/base/GPUSortingSearch.java:
+117, 203-240).  Results carry position triples exactly as the reference's
+native search returns them (emitter/src/native/jni/jni_sorter.cu:100-110).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from ._lib import (FIRST, COUNT_ALL, MOVES_ALL, MOVES_REF, BatchOptions, BatchStats,
+                   Request, Result, ServeOptions, ServeStats, Td16Error, check, lib, msgbuf)
+
+Move = Tuple[int, int, int]
+
+
+def _u8(seq: Sequence[int]):
+    return (ctypes.c_uint8 * max(1, len(seq)))(*seq)
+
+
+def make_request(beta: Sequence[int], omega: Sequence[int], initial_even: int, min_rate: float,
+                 max_moves: int, full_sorting: bool = True, moveset: int = MOVES_REF,
+                 mode: int = FIRST, iddfs: bool = False) -> Request:
+    if len(beta) != len(omega):
+        raise Td16Error(1, "beta and omega must have equal lengths")
+    if len(beta) > 17:
+        raise Td16Error(1, "performGpuSortingSearch: n exceeds MAX_N")
+    r = Request()
+    for q, x in enumerate(beta):
+        r.beta[q] = x & 0xFF
+    for q, x in enumerate(omega):
+        r.omega[q] = x & 0xFF
+    r.n = len(beta)
+    r.initial_even = int(initial_even)
+    r.min_rate = float(min_rate)
+    r.max_moves = int(max_moves)
+    r.full_sorting = 1 if full_sorting else 0
+    r.moveset = int(moveset)
+    r.mode = int(mode)
+    r.iddfs = 1 if iddfs else 0
+    return r
+
+
+def linear_request(perm: int, n: int, k: int, moveset: int = MOVES_REF, mode: int = FIRST,
+                   iddfs: bool = False) -> Request:
+    """A nibble-packed linear permutation as a request: beta = [pi, n],
+    omega(y) = pred(y) + 1, min_rate by the emitter's convention."""
+    r = Request()
+    st = lib().td16_linear_request(int(perm), n, k, moveset, mode, 1 if iddfs else 0,
+                                   ctypes.byref(r))
+    if st != 0:
+        raise Td16Error(st, "invalid linear instance")
+    return r
+
+
+# ---------------------------------------------------------------------------
+# slots and the drop-in search
+# ---------------------------------------------------------------------------
+
+def init_slots(total_slots: int, devices: Sequence[int], queue_bytes_budget: int) -> None:
+    arr = (ctypes.c_int32 * max(1, len(devices)))(*devices)
+    m = msgbuf()
+    check(lib().td16_init_slots(total_slots, arr if devices else None, len(devices),
+                                int(queue_bytes_budget), m, len(m)), m)
+
+
+def shutdown() -> None:
+    m = msgbuf()
+    check(lib().td16_shutdown(m, len(m)), m)
+
+
+def slot_count() -> int:
+    return lib().td16_slot_count()
+
+
+def slot_device(slot: int) -> int:
+    return lib().td16_slot_device(slot)
+
+
+def search_positions(slot: int, beta: Sequence[int], omega: Sequence[int], initial_even: int,
+                     min_rate: float, max_moves: int, full_sorting: bool = True) -> List[Move]:
+    """The drop-in: jni_sorter.cu:49-117 -> int[len][3] positions."""
+    if beta is None or omega is None:
+        raise Td16Error(1, "beta and omega must not be null")
+    if len(beta) != len(omega):
+        raise Td16Error(1, "beta and omega must have equal lengths")
+    out = (ctypes.c_int32 * 27)()
+    ln = ctypes.c_int32(0)
+    m = msgbuf()
+    check(lib().td16_search(slot, _u8([x & 0xFF for x in beta]), _u8([x & 0xFF for x in omega]),
+                            len(beta), int(initial_even), float(min_rate), int(max_moves),
+                            1 if full_sorting else 0, out, ctypes.byref(ln), m, len(m)), m)
+    return [(out[3 * d], out[3 * d + 1], out[3 * d + 2]) for d in range(ln.value)]
+
+
+def search_ex(slot: int, req: Request) -> Result:
+    res = Result()
+    m = msgbuf()
+    check(lib().td16_search_ex(slot, ctypes.byref(req), ctypes.byref(res), m, len(m)), m)
+    return res
+
+
+# ---------------------------------------------------------------------------
+# batched
+# ---------------------------------------------------------------------------
+
+@dataclass
+class BatchRun:
+    results: ctypes.Array
+    stats: BatchStats
+
+
+def batch(reqs: Sequence[Request], device: int = 0, waves_per_cu: int = 0, split_depth: int = 0,
+          donate_max: int = 0, use_graph: bool = False, repeats: int = 1,
+          queue_bytes: int = 0) -> BatchRun:
+    n = len(reqs)
+    arr = (Request * max(1, n))(*reqs) if not isinstance(reqs, ctypes.Array) else reqs
+    out = (Result * max(1, n))()
+    opt = BatchOptions(device, waves_per_cu, split_depth, donate_max, 1 if use_graph else 0,
+                       repeats, queue_bytes)
+    st = BatchStats()
+    m = msgbuf()
+    check(lib().td16_batch(arr, n, out, ctypes.byref(opt), ctypes.byref(st), m, len(m)), m)
+    return BatchRun(out, st)
+
+
+# ---------------------------------------------------------------------------
+# served loop
+# ---------------------------------------------------------------------------
+
+def serve_start(device: int = 0, slots: int = 64, queue_capacity: int = 512,
+                slot_bytes: int = 1 << 30, waves_per_cu: int = 0) -> None:
+    opt = ServeOptions(device, slots, queue_capacity, waves_per_cu, slot_bytes)
+    m = msgbuf()
+    check(lib().td16_serve_start(ctypes.byref(opt), m, len(m)), m)
+
+
+def serve_submit(req: Request, nonblocking: bool = False) -> int:
+    t = ctypes.c_uint64(0)
+    m = msgbuf()
+    check(lib().td16_serve_submit(ctypes.byref(req), 1 if nonblocking else 0, ctypes.byref(t),
+                                  m, len(m)), m)
+    return t.value
+
+
+def serve_wait(ticket: int) -> Result:
+    res = Result()
+    m = msgbuf()
+    check(lib().td16_serve_wait(ticket, ctypes.byref(res), m, len(m)), m)
+    return res
+
+
+def serve_search(req: Request) -> Result:
+    res = Result()
+    m = msgbuf()
+    check(lib().td16_serve_search(ctypes.byref(req), ctypes.byref(res), m, len(m)), m)
+    return res
+
+
+def serve_stats(reset_latency: bool = False) -> ServeStats:
+    st = ServeStats()
+    m = msgbuf()
+    check(lib().td16_serve_stats_get(ctypes.byref(st), 1 if reset_latency else 0, m, len(m)), m)
+    return st
+
+
+def serve_stop() -> None:
+    m = msgbuf()
+    check(lib().td16_serve_stop(m, len(m)), m)
+
+
+# ---------------------------------------------------------------------------
+# replay and certificate format
+# ---------------------------------------------------------------------------
+
+def positions_to_symbols(beta: Sequence[int], moves: Sequence[Move]) -> List[Move]:
+    flat = [x for mv in moves for x in mv]
+    pos = (ctypes.c_int32 * max(1, len(flat)))(*flat)
+    sym = (ctypes.c_int32 * max(1, len(flat)))()
+    m = msgbuf()
+    check(lib().td16_positions_to_symbols(_u8(beta), len(beta), pos, len(moves), sym, m, len(m)),
+          m)
+    return [(sym[3 * q], sym[3 * q + 1], sym[3 * q + 2]) for q in range(len(moves))]
+
+
+def validate(beta: Sequence[int], omega: Sequence[int], symbol_moves: Sequence[Move],
+             max_moves: int) -> None:
+    """Raises Td16Error (IllegalArgumentException in the reference) when invalid."""
+    flat = [x for mv in symbol_moves for x in mv]
+    arr = (ctypes.c_int32 * max(1, len(flat)))(*flat)
+    m = msgbuf()
+    check(lib().td16_validate(_u8(beta), _u8(omega), len(beta), arr, len(symbol_moves),
+                              int(max_moves), m, len(m)), m)
+
+
+def codec_key(omega: Sequence[int]) -> bytes:
+    buf = (ctypes.c_uint8 * 32)()
+    ln = ctypes.c_int32(0)
+    m = msgbuf()
+    check(lib().td16_codec_key(_u8(omega), len(omega), buf, 32, ctypes.byref(ln), m, len(m)), m)
+    return bytes(buf[:ln.value])
+
+
+def codec_value(symbol_moves: Sequence[Move]) -> bytes:
+    flat = [x for mv in symbol_moves for x in mv]
+    arr = (ctypes.c_int32 * max(1, len(flat)))(*flat)
+    buf = (ctypes.c_uint8 * 64)()
+    ln = ctypes.c_int32(0)
+    m = msgbuf()
+    check(lib().td16_codec_value(arr, len(symbol_moves), buf, 64, ctypes.byref(ln), m, len(m)), m)
+    return bytes(buf[:ln.value])
+
+
+def codec_decode_key(data: bytes, n: int) -> List[int]:
+    om = (ctypes.c_uint8 * 32)()
+    m = msgbuf()
+    check(lib().td16_codec_decode_key(_u8(list(data)), len(data), n, om, m, len(m)), m)
+    return list(om[:n])
+
+
+def codec_decode_value(data: bytes) -> List[Move]:
+    sym = (ctypes.c_int32 * 96)()
+    ln = ctypes.c_int32(0)
+    m = msgbuf()
+    check(lib().td16_codec_decode_value(_u8(list(data)), len(data), sym, 32, ctypes.byref(ln),
+                                        m, len(m)), m)
+    return [(sym[3 * q], sym[3 * q + 1], sym[3 * q + 2]) for q in range(ln.value)]
+
+
+# ---------------------------------------------------------------------------
+# synthetic instances
+# ---------------------------------------------------------------------------
+
+def gen_uniform(seed: int, n: int, count: int) -> np.ndarray:
+    out = np.zeros(count, dtype=np.uint64)
+    st = lib().td16_gen_uniform(seed, n, count, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    if st != 0:
+        raise Td16Error(st, "invalid generator arguments")
+    return out
+
+
+def gen_products(seed: int, nmin: int, nmax: int, kmin: int, kmax: int, count: int):
+    ns = np.zeros(count, dtype=np.uint8)
+    ks = np.zeros(count, dtype=np.uint8)
+    perms = np.zeros(count, dtype=np.uint64)
+    st = lib().td16_gen_products(seed, nmin, nmax, kmin, kmax, count,
+                                 ns.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                 ks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                 perms.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    if st != 0:
+        raise Td16Error(st, "invalid generator arguments")
+    return ns, ks, perms
+
+
+def gen_exponential(seed: int, count: int) -> np.ndarray:
+    out = np.zeros(count, dtype=np.float64)
+    lib().td16_gen_exponential(seed, count, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def version() -> str:
+    return lib().td16_version().decode()
+
+
+__all__ = [n for n in dir() if not n.startswith("_")] + ["FIRST", "COUNT_ALL", "MOVES_REF",
+                                                        "MOVES_ALL"]

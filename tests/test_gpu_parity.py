@@ -1,0 +1,103 @@
+"""GPU parity: the HIP search (through the C ABI) against the CPU oracle.
+
+Bars: found / not-found, witness replay validity, and -- for searches that a
+single wavefront ran from start to finish (result.split == 0) or that were
+exhaustive -- bit-exact witnesses and node counts.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def slots(native):
+    native.init_slots(2, [0], 64 << 20)
+    yield
+    native.shutdown()
+
+
+def pair_of(req):
+    return list(req.beta[:req.n]), list(req.omega[:req.n])
+
+
+def check_against_oracle(O, native, req, res, exact_counts=False):
+    beta, omega = pair_of(req)
+    if req.iddfs:
+        d, ref = O.iddfs(beta, omega, req.moveset, req.max_moves)
+        assert res.found == (d >= 0)
+        if d >= 0:
+            assert res.depth == d
+    else:
+        ref = O.search(beta, omega, req.initial_even, req.min_rate, req.max_moves,
+                       bool(req.full_sorting), req.moveset, req.mode)
+        assert res.found == ref.found
+    if res.found and res.depth > 0:
+        sym = native.positions_to_symbols(beta, res.witness())
+        assert O.validate(beta, omega, sym, req.max_moves) == 0
+    if req.mode == 1 or not res.found:
+        assert res.expanded == ref.expanded
+        assert res.solutions == ref.solutions
+    elif not res.split:
+        assert res.witness() == ref.witness()
+        assert res.expanded == ref.expanded
+
+
+def test_reference_smoke_vector(native, slots):
+    # GPUSortingSearchSmokeTest.java:26-35
+    moves = native.search_positions(0, [0, 1, 2], [1, 2, 0], 1, 2.0, 1, True)
+    assert moves == [(0, 1, 2)]
+
+
+def test_plain_against_oracle(O, native, slots):
+    ns, ks, perms = native.gen_products(11, 8, 16, 3, 9, 48)
+    for q in range(len(ns)):
+        req = native.linear_request(int(perms[q]), int(ns[q]), int(ks[q]))
+        res = native.search_ex(q % 2, req)
+        check_against_oracle(O, native, req, res)
+
+
+def test_batch_config2_sample(O, native):
+    ns, ks, perms = native.gen_products(1, 12, 16, 4, 9, 400)
+    reqs = [native.linear_request(int(perms[q]), int(ns[q]), int(ks[q])) for q in range(400)]
+    run = native.batch(reqs)
+    for q in range(400):
+        check_against_oracle(O, native, reqs[q], run.results[q])
+
+
+def bounded_request(native, perm, n, slack, moveset, mode):
+    """COUNT_ALL-sized request: move bound = odd-cycle bound + slack."""
+    base = native.linear_request(int(perm), int(n), 9, moveset, mode)
+    lb = (base.n - base.initial_even) // 2
+    D = min(9, lb + slack)
+    beta, omega = pair_of(base)
+    return native.make_request(beta, omega, base.initial_even, 2.0 / (D / float(lb)), D, True,
+                               moveset, mode)
+
+
+def test_batch_count_all(O, native):
+    ns, ks, perms = native.gen_products(5, 8, 11, 4, 6, 64)
+    reqs = []
+    for q in range(64):
+        for ms in (native.MOVES_REF, native.MOVES_ALL):
+            reqs.append(bounded_request(native, perms[q], ns[q], 0, ms, native.COUNT_ALL))
+    ns, ks, perms = native.gen_products(6, 6, 8, 3, 5, 32)
+    for q in range(32):
+        reqs.append(bounded_request(native, perms[q], ns[q], 1, native.MOVES_REF, native.COUNT_ALL))
+    run = native.batch(reqs)
+    for q, req in enumerate(reqs):
+        check_against_oracle(O, native, req, run.results[q], exact_counts=True)
+
+
+def test_batch_iddfs_all_n7(O, native):
+    _, dist, _ = O.bfs(7)
+    reqs = []
+    for r in range(1, math.factorial(7)):
+        p = O.perm_unrank(r, 7)
+        reqs.append(native.linear_request(O.pack(p), 7, 9, native.MOVES_ALL, native.FIRST, True))
+    run = native.batch(reqs)
+    for q in range(len(reqs)):
+        assert run.results[q].found == 1
+        assert run.results[q].depth == dist[q + 1]

@@ -152,7 +152,7 @@ typedef struct {
     int32_t device;          /* HIP device ordinal */
     int32_t waves_per_cu;    /* 0 = occupancy maximum */
     int32_t split_depth;     /* frames deeper than max_moves - split_depth are
-                                never donated; 0 = default (3) */
+                                never donated; 0 = default (5) */
     int32_t donate_max;      /* children donated per donation; 0 = default (64) */
     int32_t use_graph;       /* 1 = launch through a captured hipGraph */
     int32_t repeats;         /* run the same batch this many times (>= 1) */
@@ -208,6 +208,13 @@ int td16_serve_wait(uint64_t ticket, td16_result *res, char *msg, int msg_cap);
 /* Submit + wait. */
 int td16_serve_search(const td16_request *req, td16_result *res, char *msg, int msg_cap);
 int td16_serve_stats_get(td16_serve_stats *st, int reset_latency, char *msg, int msg_cap);
+/* Open-loop load: submits reqs[q] at arrival_us[q] microseconds after the
+ * call starts (spinning until then), waits for every result, and resets the
+ * latency window first so that td16_serve_stats_get reports this replay's
+ * p50/p99, measured from each scheduled arrival to the result's arrival on
+ * the host.  *elapsed_us receives the replay's wall time. */
+int td16_serve_replay(const td16_request *reqs, const double *arrival_us, int count,
+                      td16_result *results, double *elapsed_us, char *msg, int msg_cap);
 /* Stops the kernel after the admitted requests finish. */
 int td16_serve_stop(char *msg, int msg_cap);
 
@@ -263,6 +270,12 @@ int td16_gen_exponential(uint64_t seed, int count, double *gaps);
  * (the emitter's convention, UnorientedCycles.java:34), max_moves = k. */
 int td16_linear_request(uint64_t perm, int n, int k, int moveset, int mode, int iddfs,
                         td16_request *req);
+
+/* Sustained int32 VALU rate of the device, in 10^12 lane-ops per second:
+ * a grid of resident wavefronts runs independent dependent chains of
+ * v_add_u32 / v_xor_b32 for `iterations` rounds.  Measurement aid for the
+ * search kernel's integer-op roofline (the search issues no MFMA). */
+int td16_probe_valu_peak(int device, int iterations, double *tops, char *msg, int msg_cap);
 
 /* Library version string. */
 const char *td16_version(void);

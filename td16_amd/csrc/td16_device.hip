@@ -95,6 +95,17 @@ __device__ __forceinline__ bool cas(T *p, T expected, T desired) {
                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <class T>
+__device__ __forceinline__ bool cas_rlx(T *p, T expected, T desired) {
+  return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Agent-scope fences.  On gfx950 each XCD has its own L2, so these cost an
+// L2 write-back (release) or invalidate (acquire): they are placed only where
+// one wavefront hands data to another (a donated item, a finished request),
+// never in polling loops.
+__device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+template <class T>
 __device__ __forceinline__ T ld_sys_acq(T *p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
@@ -170,6 +181,8 @@ struct Ctx {
   uint64_t expanded;
   uint64_t solutions;
   int sinceCheck;
+  int home;        // this wave's queue shard
+  uint32_t rng;    // LCG state for random shard probes
 };
 
 enum { kExpandOk = 0, kExpandFound = 1 };
@@ -177,7 +190,8 @@ enum { kExpandOk = 0, kExpandFound = 1 };
 // Packs the witness into the job state when this wave is the first finder.
 __device__ void record_witness(const Params &P, Ctx &c, WaveLds &w, int depth, int lastMove) {
   JobState *st = &P.states[c.job];
-  const int won = lane0([&] { return cas(&st->found, 0, 1) ? 1 : 0; });
+  // the witness is published by the release on `pending` at the item's end
+  const int won = lane0([&] { return cas_rlx(&st->found, 0, 1) ? 1 : 0; });
   if (!won) return;
   const int lane = lane_id();
   if (lane < depth) {
@@ -369,9 +383,13 @@ __device__ __forceinline__ int frame_move(const Frame &F, const uint16_t *lex, i
 // Shared queue
 // ---------------------------------------------------------------------------
 
-// Tries to donate the still-unvisited children of the shallowest frame that
-// has any (within the split depth).  Returns the number donated.
-__device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, int d) {
+// Tries to donate up to `want` still-unvisited children of the shallowest
+// frame that has any (within the split depth).  Each donating lane pushes its
+// own child into its own ring shard (home shard + its rank), so one donation
+// lands on several shards and no tail pointer sees more than one CAS from
+// it.  Children whose push fails stay with this wave.  Returns the number
+// donated.
+__device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, int d, int want) {
   const int lane = lane_id();
   const int maxFd = c.limit - P.splitSlack;
   int fd = -1, ch = -1;
@@ -392,38 +410,48 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   if (fd < 0) return 0;
   Frame &F = w.frames[fd];
   uint64_t bits = uni64(F.mask[ch]);
-  // keep at most donateMax of them (lowest bits go)
-  int cnt = __builtin_popcountll(bits);
-  if (cnt > P.donateMax) {
+  // keep at most `want` of them (the lowest bits go)
+  if (want < 1) want = 1;
+  if (__builtin_popcountll(bits) > want) {
     uint64_t keep = bits;
-    for (int q = 0; q < P.donateMax; ++q) keep &= keep - 1;
+    for (int q = 0; q < want; ++q) keep &= keep - 1;
     bits ^= keep;
-    cnt = P.donateMax;
   }
-  // reserve cnt consecutive ring slots
   const uint64_t cap = P.qcap;
-  int ok = 0;
-  uint64_t t0 = 0;
-  for (int tries = 0; tries < 4 && !ok; ++tries) {
-    t0 = uni64(lane == 0 ? ld_rlx(&P.queue->tail) : 0);
-    bool busy = false;
-    if (lane < cnt) {
-      const uint64_t pos = t0 + (uint64_t)lane;
-      busy = ld_acq(&P.seq[pos % cap]) != pos;
+  const bool mine = (bits >> lane) & 1ULL;
+  int shard = 0;
+  uint64_t pos = 0;
+  bool ok = false;
+  if (mine) {
+    shard = (c.home + rank_below(bits)) % kQueueShards;
+    QueueState *q = &P.queue[shard];
+    uint64_t *sq = P.seq + (uint64_t)shard * cap;
+    for (int tries = 0; tries < 2 && !ok; ++tries) {
+      const uint64_t t = ld_rlx(&q->tail);
+      const uint64_t s = ld_rlx(&sq[t % cap]);
+      if (s == t) {
+        if (cas_rlx(&q->tail, t, t + 1)) {
+          ok = true;
+          pos = t;
+        }
+      } else if ((int64_t)(s - t) < 0) {
+        break;  // shard full
+      }
     }
-    if (ballot(busy) != 0) return 0;  // ring full (or racing): keep working locally
-    ok = lane0([&] { return cas(&P.queue->tail, t0, t0 + (uint64_t)cnt) ? 1 : 0; });
   }
-  if (!ok) return 0;
-  // account before publishing so the request cannot complete underneath
+  const uint64_t okBits = ballot(ok);
+  const int cnt = __builtin_popcountll(okBits);
+  if (cnt == 0) return 0;
+  // account before publishing so the request cannot complete underneath;
+  // the release fence below orders these before any consumer's claim
   if (lane == 0) {
-    add_ar(&P.states[c.job].pending, cnt);
+    add_rlx(&P.states[c.job].pending, cnt);
     st_rlx(&P.states[c.job].split, 1);
     add_rlx(&P.ctrl->donations, (uint64_t)cnt);
+    add_rlx(&P.ctrl->queued, (int64_t)cnt);
   }
-  const bool mine = (bits >> lane) & 1ULL;
-  if (mine) {
-    const int slot = rank_below(bits);
+  const uint64_t slot = (uint64_t)shard * cap + pos % cap;
+  if (ok) {
     const int mv = frame_move_lane(F, lex, ch, lane);
     Item it;
     it.pi = apply_move(F.pi, mv & 31, (mv >> 5) & 31, (mv >> 10) & 31);
@@ -433,46 +461,48 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
     for (int q = 0; q < 10; ++q)
       it.moves[q] = (q < fd) ? w.path[q] : (q == fd ? (uint16_t)mv : (uint16_t)0);
     it.pad[0] = it.pad[1] = it.pad[2] = 0;
-    const uint64_t pos = t0 + (uint64_t)slot;
-    P.items[pos % cap] = it;
-    __threadfence();
-    st_rel(&P.seq[pos % cap], pos + 1);
+    P.items[slot] = it;
   }
-  if (lane == 0) F.mask[ch] &= ~bits;
+  fence_rel();  // one L2 write-back publishes every item of this donation
+  if (ok) st_rlx(&P.seq[slot], pos + 1);
+  if (lane == 0) F.mask[ch] &= ~okBits;
   wave_sync();
   c.shared = true;
   return cnt;
 }
 
-// Pops one item; returns 1 and fills `it` when there was one.
-__device__ int pop_item(const Params &P, Item &it, WaveLds &w) {
+// Claims one item: the home shard first, then one pseudo-random shard.
+// Returns 1 and fills `it` (and the wave's path prefix) when there was one.
+__device__ int pop_item(const Params &P, Ctx &c, Item &it, WaveLds &w) {
   const int lane = lane_id();
   const uint64_t cap = P.qcap;
-  int got = 0;
+  c.rng = c.rng * 1664525u + 1013904223u;
+  const int other = (int)((c.rng >> 8) % kQueueShards);
+  int got = 0, shard = 0;
   uint64_t h = 0;
   if (lane == 0) {
-    for (int tries = 0; tries < 8; ++tries) {
-      h = ld_rlx(&P.queue->head);
-      const uint64_t s = ld_acq(&P.seq[h % cap]);
-      const int64_t dif = (int64_t)(s - (h + 1));
-      if (dif == 0) {
-        if (cas(&P.queue->head, h, h + 1)) {
-          got = 1;
-          break;
-        }
-      } else if (dif < 0) {
-        break;  // empty
-      }
+    // relaxed polling: the acquire fence is paid only once a slot is claimed
+    for (int probe = 0; probe < 2 && !got; ++probe) {
+      shard = probe == 0 ? c.home : other;
+      QueueState *q = &P.queue[shard];
+      const uint64_t *sq = P.seq + (uint64_t)shard * cap;
+      h = ld_rlx(&q->head);
+      const uint64_t s = ld_rlx(&sq[h % cap]);
+      if (s == h + 1 && cas_rlx(&q->head, h, h + 1)) got = 1;
     }
   }
   got = uni(got);
   if (!got) return 0;
+  shard = uni(shard);
   h = uni64(h);
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
-  it = P.items[h % cap];
-  if (lane < 10) w.path[lane] = P.items[h % cap].moves[lane];
-  __threadfence();
-  if (lane == 0) st_rel(&P.seq[h % cap], h + cap);
+  fence_acq();
+  const uint64_t slot = (uint64_t)shard * cap + h % cap;
+  it = P.items[slot];
+  if (lane < 10) w.path[lane] = P.items[slot].moves[lane];
+  if (lane == 0) add_rlx(&P.ctrl->queued, (int64_t)-1);
+  // the item is in registers once every load has returned: free the slot
+  __builtin_amdgcn_s_waitcnt(0);
+  if (lane == 0) st_rlx(&P.seq[slot], h + cap);
   wave_sync();
   return 1;
 }
@@ -482,14 +512,13 @@ __device__ int pop_item(const Params &P, Item &it, WaveLds &w) {
 // ---------------------------------------------------------------------------
 
 // Loads the request's parameters into the wave context and LDS tables.
-__device__ void load_job(const Params &P, Ctx &c, WaveLds &w, uint32_t j) {
+__device__ void load_job(const Params &P, Ctx &c, WaveLds &w, uint32_t j, int limit) {
   const int lane = lane_id();
   const Job *job = &P.jobs[j];
   c.job = j;
   c.N = uni(job->N);
   c.n = c.N - 1;
   c.flags = uni(job->flags);
-  const int limit = uni(ld_rlx(&P.states[j].limit));
   c.limit = limit;
   c.Rreg = lane < 20 ? (int)job->R[lane < 20 ? lane : 0] : 0;
   if (lane <= kMaxDepth) {
@@ -550,11 +579,25 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
       w.path[d] = (uint16_t)mv;
     }
     wave_sync();
-    // sharing and cancellation checks
-    if (++c.sinceCheck >= 16) {
+    // sharing and cancellation checks.  Subtrees are donated only when
+    // wavefronts sit idle, no unstarted request is waiting for them and this
+    // item has proved heavy: while requests remain, request-level
+    // parallelism keeps every wavefront busy for free.
+    if (++c.sinceCheck >= 32) {
       c.sinceCheck = 0;
-      const int idle = lane0([&] { return ld_rlx(&P.ctrl->idle); });
-      if (idle > 0) donate(P, c, w, lex, d);
+      const int want = lane0([&] {
+        const int idle = ld_rlx(&P.ctrl->idle);
+        if (idle <= 0) return 0;
+        const bool drained = P.served
+                                 ? ld_rlx(&P.ctrl->consumed) >= ld_rlx(&P.ctrl->submittedMirror)
+                                 : ld_rlx(&P.ctrl->nextJob) >= P.ctrl->numJobs;
+        if (!drained) return 0;
+        // donate only what idle wavefronts are not already being offered
+        const int64_t demand = (int64_t)idle - ld_rlx(&P.ctrl->queued);
+        if (demand <= 0) return 0;
+        return demand < P.donateMax ? (int)demand : P.donateMax;
+      });
+      if (want > 0) donate(P, c, w, lex, d, want);
       if (c.shared && !countAll) {
         if (lane0([&] { return ld_rlx(&st->found); }) != 0) break;
       }
@@ -562,20 +605,25 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
     ++d;
     if (expand(P, c, w, lex, d, child) == kExpandFound) break;
   }
-  // flush counters, then release this item
+  // flush counters, then release this item (the release publishes the
+  // counters and any witness to the wavefront that finalizes the request)
   int last = 0;
   if (lane == 0) {
     if (c.expanded) add_rlx(&st->expanded, c.expanded);
     if (c.solutions) add_rlx(&st->solutions, c.solutions);
-    last = add_ar(&st->pending, -1) == 1 ? 1 : 0;
+    last = __hip_atomic_fetch_add(&st->pending, -1, __ATOMIC_RELEASE,
+                                  __HIP_MEMORY_SCOPE_AGENT) == 1 ? 1 : 0;
   }
-  return uni(last) != 0;
+  last = uni(last);
+  if (last && (c.shared || uni(ld_rlx(&st->split)))) fence_acq();
+  return last != 0;
 }
 
 __device__ void publish_result(const Params &P, uint32_t j) {
   const int lane = lane_id();
   JobState *st = &P.states[j];
-  __atomic_thread_fence(__ATOMIC_ACQUIRE);
+  // the caller already acquired what other wavefronts wrote; the host reads
+  // batched results after the kernel has ended
   if (!P.served) {
     if (lane == 0) {
       DevResult r;
@@ -590,7 +638,6 @@ __device__ void publish_result(const Params &P, uint32_t j) {
       r.seq = 0;
       r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0;
       P.results[j] = r;
-      __threadfence();
       add_rlx(&P.ctrl->jobsDone, 1u);
     }
   } else {
@@ -618,21 +665,25 @@ __device__ void publish_result(const Params &P, uint32_t j) {
   }
 }
 
-// Starts request j (root item) in this wave.
-__device__ void start_job(const Params &P, uint32_t j) {
+// Starts request j (root item) in this wave; returns its first depth limit.
+__device__ int start_job(const Params &P, uint32_t j) {
   const int lane = lane_id();
+  const Job *job = &P.jobs[j];
+  const int limit = uni((job->flags & kFlagIddfs) ? job->firstLimit : job->maxDepth);
   if (lane == 0) {
     JobState *st = &P.states[j];
-    const Job *job = &P.jobs[j];
     st->pending = 1;
     st->found = 0;
-    st->limit = (job->flags & kFlagIddfs) ? job->firstLimit : job->maxDepth;
+    st->limit = limit;
     st->split = 0;
     st->expanded = 0;
     st->solutions = 0;
     st->depth = 0;
   }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+  // no fence: until this wave donates (which releases), no other wavefront
+  // reads this state
+  wave_sync();
+  return limit;
 }
 
 // Served loop: copies a newly submitted request into the device job table.
@@ -644,19 +695,22 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut) {
     uint64_t avail = ld_rlx(&P.ctrl->submittedMirror);
     r = ld_rlx(&P.ctrl->consumed);
     if (r >= avail) {
-      // refresh the mirror from host memory at most every ~2 us
-      const uint64_t now = __builtin_readcyclecounter();
+      // refresh the mirror from host memory at most every ~2 us, timed by
+      // the chip-wide 100 MHz real-time counter
+      const uint64_t now = __builtin_amdgcn_s_memrealtime();
       const uint64_t last = ld_rlx(&P.ctrl->lastPoll);
-      if (now - last > 4000ULL && cas(&P.ctrl->lastPoll, last, now)) {
+      if (now - last > 200ULL && cas_rlx(&P.ctrl->lastPoll, last, now)) {
+        // stop is read first: the host raises it only after its last submit
+        if (ld_sys_acq(&P.host->stop) != 0) st_rlx(&P.ctrl->stopMirror, (uint64_t)1);
         avail = ld_sys_acq(&P.host->submitted);
         uint64_t cur = ld_rlx(&P.ctrl->submittedMirror);
-        while (cur < avail && !cas(&P.ctrl->submittedMirror, cur, avail))
+        while (cur < avail && !cas_rlx(&P.ctrl->submittedMirror, cur, avail))
           cur = ld_rlx(&P.ctrl->submittedMirror);
       }
       avail = ld_rlx(&P.ctrl->submittedMirror);
     }
     while (r < avail) {
-      if (cas(&P.ctrl->consumed, r, r + 1)) {
+      if (cas_rlx(&P.ctrl->consumed, r, r + 1)) {
         got = 1;
         break;
       }
@@ -666,6 +720,8 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut) {
   got = uni(got);
   if (!got) return 0;
   r = uni64(r);
+  // synchronize with the host's release of `submitted` before reading entry r
+  if (lane == 0) (void)ld_sys_acq(&P.host->submitted);
   // ring entries are transport only: the record's tag names its job slot
   const uint32_t e = (uint32_t)(r & (uint64_t)(P.ringCap - 1));
   const uint32_t *src = reinterpret_cast<const uint32_t *>(&P.hostJobs[e]);
@@ -698,12 +754,14 @@ td16_search_kernel(Params P) {
 
   Ctx c;
   c.shared = false;
+  c.home = (int)((blockIdx.x * kWavesPerBlock + wid) % kQueueShards);
+  c.rng = 0x9E3779B9u * (blockIdx.x * kWavesPerBlock + wid + 1);
   bool idle = false;
   int backoff = 1;
   for (;;) {
     Item it;
     uint32_t j = 0;
-    bool haveItem = pop_item(P, it, w) != 0;
+    bool haveItem = pop_item(P, c, it, w) != 0;
     bool haveJob = false;
     if (!haveItem) {
       if (!P.served) {
@@ -730,16 +788,22 @@ td16_search_kernel(Params P) {
       if (!P.served) {
         done = lane0([&] { return ld_rlx(&P.ctrl->jobsDone) >= P.ctrl->numJobs ? 1 : 0; });
       } else {
+        // device-side mirrors only: host memory is read by the rate-limited
+        // poll in take_submitted, never by every idle wavefront
         done = lane0([&] {
-          if (ld_sys_acq(&P.host->stop) == 0) return 0;
+          if (ld_rlx(&P.ctrl->stopMirror) == 0) return 0;
           const uint64_t cons = ld_rlx(&P.ctrl->consumed);
-          const uint64_t sub = ld_sys_acq(&P.host->submitted);
-          return (cons >= sub && ld_rlx(&P.ctrl->jobsDone) >= (uint32_t)cons) ? 1 : 0;
+          return (cons >= ld_rlx(&P.ctrl->submittedMirror) &&
+                  ld_rlx(&P.ctrl->jobsDone) >= (uint32_t)cons) ? 1 : 0;
         });
       }
       if (done) break;
-      for (int q = 0; q < backoff; ++q) __builtin_amdgcn_s_sleep(2);
-      backoff = backoff < 16 ? backoff * 2 : 16;
+      // idle polling touches a few words of HBM; back off to keep those
+      // cache lines cool (about 0.4 us per unit, up to ~14 us batched and
+      // ~3.5 us served)
+      for (int q = 0; q < backoff; ++q) __builtin_amdgcn_s_sleep(16);
+      const int maxBackoff = P.served ? 8 : 32;
+      backoff = backoff < maxBackoff ? backoff * 2 : maxBackoff;
       continue;
     }
     if (idle) {
@@ -749,9 +813,9 @@ td16_search_kernel(Params P) {
     backoff = 1;
 
     uint64_t pi;
-    int depth0;
+    int depth0, limit;
     if (haveJob) {
-      start_job(P, j);
+      limit = start_job(P, j);
       pi = uni64(P.jobs[j].pi0);
       depth0 = 0;
       c.shared = false;
@@ -759,29 +823,30 @@ td16_search_kernel(Params P) {
       j = (uint32_t)uni((int)it.job);
       pi = uni64(it.pi);
       depth0 = uni((int)it.depth);
+      limit = uni(ld_rlx(&P.states[j].limit));  // ordered by pop_item's acquire
       c.shared = true;
     }
-    load_job(P, c, w, j);
+    load_job(P, c, w, j, limit);
     for (;;) {
       if (!run_item(P, c, w, lex, pi, depth0)) break;
       // last item of request j finished: next IDDFS limit or publish
       JobState *st = &P.states[j];
-      const int again = lane0([&] {
-        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+      const int next = lane0([&] {
         const Job *job = &P.jobs[j];
         if ((job->flags & kFlagIddfs) && st->found == 0 && st->limit < job->maxDepth) {
-          st->limit = st->limit + 1;
+          const int nl = st->limit + 1;
+          st->limit = nl;
           st->pending = 1;
-          __threadfence();
-          return 1;
+          return nl;
         }
         return 0;
       });
-      if (!again) {
+      wave_sync();
+      if (next == 0) {
         publish_result(P, j);
         break;
       }
-      load_job(P, c, w, j);
+      load_job(P, c, w, j, next);
       pi = uni64(P.jobs[j].pi0);
       depth0 = 0;
       c.shared = false;
@@ -790,9 +855,10 @@ td16_search_kernel(Params P) {
   if (idle && lane == 0) add_rlx(&P.ctrl->idle, -1);
 }
 
-__global__ void td16_queue_init_kernel(uint64_t *seq, uint64_t cap) {
+// Every shard's slot q starts out free for producer position q.
+__global__ void td16_queue_init_kernel(uint64_t *seq, uint64_t capPerShard, uint64_t total) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < cap) seq[i] = i;
+  if (i < total) seq[i] = i % capPerShard;
 }
 
 int launch_search(const Params &p, int blocks, void *stream) {
@@ -801,10 +867,11 @@ int launch_search(const Params &p, int blocks, void *stream) {
   return (int)hipGetLastError();
 }
 
-int launch_queue_init(uint64_t *seq, uint64_t cap, void *stream) {
-  const uint64_t blocks = (cap + 255) / 256;
+int launch_queue_init(uint64_t *seq, uint64_t capPerShard, void *stream) {
+  const uint64_t total = capPerShard * kQueueShards;
+  const uint64_t blocks = (total + 255) / 256;
   hipLaunchKernelGGL(td16_queue_init_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                     (hipStream_t)stream, seq, cap);
+                     (hipStream_t)stream, seq, capPerShard, total);
   return (int)hipGetLastError();
 }
 

@@ -84,9 +84,12 @@ struct alignas(16) Item {
 };
 static_assert(sizeof(Item) == 48, "Item layout");
 
-// Shared work queue: a bounded lock-free ring (per-slot sequence numbers,
-// D. Vyukov's bounded MPMC scheme).  head/tail only grow and persist across
-// launches; every launch ends with the queue empty.
+// Shared work queue: kQueueShards bounded lock-free rings (per-slot sequence
+// numbers, D. Vyukov's bounded MPMC scheme), so that idle wavefronts poll and
+// claim on different cache lines.  Shard s owns ring slots
+// [s*cap, (s+1)*cap).  head/tail only grow and persist across launches;
+// every launch ends with every shard empty.
+constexpr int kQueueShards = 64;
 struct alignas(128) QueueState {
   uint64_t head;
   uint8_t pad0[120];
@@ -105,12 +108,15 @@ struct alignas(128) Control {
   uint32_t pad2[31];
   uint64_t donations;
   uint64_t pad3[15];
+  int64_t queued;        // items published and not yet claimed (all shards)
+  uint64_t pad5[15];
   // served loop
   uint64_t consumed;     // submit-ring entries taken by the device
   uint64_t completions;  // completion-ring entries written
   uint64_t lastPoll;     // device clock of the last host-ring poll
   uint64_t submittedMirror;
-  uint64_t pad4[12];
+  uint64_t stopMirror;   // device copy of ServeHost::stop
+  uint64_t pad4[11];
 };
 
 // Host-visible rings of the served loop (fine-grained pinned host memory).
@@ -126,10 +132,10 @@ struct Params {
   JobState *states;
   DevResult *results;
   Control *ctrl;
-  QueueState *queue;
+  QueueState *queue;     // kQueueShards shards
   Item *items;
   uint64_t *seq;
-  uint64_t qcap;
+  uint64_t qcap;         // slots per shard
   const uint16_t *lex;   // lexicographic triple tables, kLexEntries entries
   int32_t splitSlack;    // donate only frames with depth <= limit - splitSlack
   int32_t donateMax;
@@ -143,7 +149,7 @@ struct Params {
 
 // Launch wrappers (td16_device.hip).
 int launch_search(const Params &p, int blocks, void *stream);
-int launch_queue_init(uint64_t *seq, uint64_t cap, void *stream);
+int launch_queue_init(uint64_t *seq, uint64_t capPerShard, void *stream);
 int search_kernel_occupancy(int *blocksPerCU);
 
 // Host helpers (td16_host.cpp).

@@ -121,23 +121,25 @@ int alloc_buffers(Buffers &b, int device, uint32_t jobsCap, int64_t queueBytes, 
   b.device = device;
   b.jobsCap = jobsCap;
   TD16_HIP(hipSetDevice(device));
-  uint64_t qcap = (uint64_t)queueBytes / (sizeof(Item) + sizeof(uint64_t));
-  if (qcap < 1024) qcap = 1024;
+  // the budget is split evenly over the ring shards
+  uint64_t qcap = (uint64_t)queueBytes / (sizeof(Item) + sizeof(uint64_t)) / kQueueShards;
+  if (qcap < 64) qcap = 64;
   b.qcap = qcap;
+  const uint64_t slots = qcap * kQueueShards;
   TD16_HIP(hipMalloc(&b.dJobs, sizeof(Job) * jobsCap));
   TD16_HIP(hipMalloc(&b.dStates, sizeof(JobState) * jobsCap));
   TD16_HIP(hipMalloc(&b.dResults, sizeof(DevResult) * jobsCap));
   TD16_HIP(hipMalloc(&b.dCtrl, sizeof(Control)));
-  TD16_HIP(hipMalloc(&b.dQueue, sizeof(QueueState)));
-  TD16_HIP(hipMalloc(&b.dItems, sizeof(Item) * qcap));
-  TD16_HIP(hipMalloc(&b.dSeq, sizeof(uint64_t) * qcap));
+  TD16_HIP(hipMalloc(&b.dQueue, sizeof(QueueState) * kQueueShards));
+  TD16_HIP(hipMalloc(&b.dItems, sizeof(Item) * slots));
+  TD16_HIP(hipMalloc(&b.dSeq, sizeof(uint64_t) * slots));
   TD16_HIP(hipHostMalloc(&b.hCtrl, sizeof(Control), hipHostMallocDefault));
   TD16_HIP(hipHostMalloc(&b.hJobs, sizeof(Job) * jobsCap, hipHostMallocDefault));
   TD16_HIP(hipHostMalloc(&b.hResults, sizeof(DevResult) * jobsCap, hipHostMallocDefault));
   std::memset(b.hCtrl, 0, sizeof(Control));
   // queue counters start at zero, slot q expects producer position q
-  static const QueueState zeroQueue{};
-  TD16_HIP(hipMemcpyAsync(b.dQueue, &zeroQueue, sizeof(QueueState), hipMemcpyHostToDevice, stream));
+  static const QueueState zeroQueue[kQueueShards] = {};
+  TD16_HIP(hipMemcpyAsync(b.dQueue, zeroQueue, sizeof(zeroQueue), hipMemcpyHostToDevice, stream));
   TD16_HIP((hipError_t)launch_queue_init(b.dSeq, qcap, stream));
   TD16_HIP(hipStreamSynchronize(stream));
   return TD16_OK;
@@ -155,7 +157,7 @@ Params make_params(const Buffers &b, const DeviceInfo &dev) {
   P.seq = b.dSeq;
   P.qcap = b.qcap;
   P.lex = dev.dLex;
-  P.splitSlack = 3;
+  P.splitSlack = 5;
   P.donateMax = 64;
   P.served = 0;
   return P;
@@ -742,8 +744,23 @@ int td16_serve_start(const td16_serve_options *opt, char *msg, int cap) {
   return TD16_OK;
 }
 
+namespace td16 {
+namespace {
+int serve_submit_at(const td16_request *req, int nonblocking, uint64_t *ticket,
+                    std::chrono::steady_clock::time_point arrival, char *msg, int cap);
+}
+}  // namespace td16
+
 int td16_serve_submit(const td16_request *req, int nonblocking, uint64_t *ticket, char *msg,
                       int cap) {
+  return td16::serve_submit_at(req, nonblocking, ticket, std::chrono::steady_clock::now(), msg,
+                               cap);
+}
+
+namespace td16 {
+namespace {
+int serve_submit_at(const td16_request *req, int nonblocking, uint64_t *ticket,
+                    std::chrono::steady_clock::time_point arrival, char *msg, int cap) {
   Server *S = gServer;
   if (S == nullptr) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
   if (req == nullptr || ticket == nullptr)
@@ -755,7 +772,7 @@ int td16_serve_submit(const td16_request *req, int nonblocking, uint64_t *ticket
   std::unique_lock<std::mutex> lk(S->mu);
   const uint64_t t = S->nextTicket++;
   Ticket &tk = S->tickets[t];
-  tk.t0 = std::chrono::steady_clock::now();
+  tk.t0 = arrival;
   *ticket = t;
   S->nSubmitted++;
   if (trivial) {
@@ -780,6 +797,8 @@ int td16_serve_submit(const td16_request *req, int nonblocking, uint64_t *ticket
   S->cvAdmit.notify_one();
   return TD16_OK;
 }
+}  // namespace
+}  // namespace td16
 
 int td16_serve_wait(uint64_t ticket, td16_result *res, char *msg, int cap) {
   Server *S = gServer;
@@ -824,6 +843,36 @@ int td16_serve_stats_get(td16_serve_stats *out, int resetLatency, char *msg, int
     out->mean_us = s / (double)v.size();
   }
   if (resetLatency) S->latUs.clear();
+  return TD16_OK;
+}
+
+int td16_serve_replay(const td16_request *reqs, const double *arrivalUs, int count,
+                      td16_result *results, double *elapsedUs, char *msg, int cap) {
+  if (gServer == nullptr)
+    return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
+  if (count < 0 || (count > 0 && (reqs == nullptr || arrivalUs == nullptr || results == nullptr)))
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid replay arguments");
+  td16_serve_stats tmp;
+  td16_serve_stats_get(&tmp, 1, nullptr, 0);  // reset the latency window
+  std::vector<uint64_t> tickets((size_t)count);
+  const auto start = std::chrono::steady_clock::now();
+  for (int q = 0; q < count; ++q) {
+    const auto due = start + std::chrono::nanoseconds((int64_t)(arrivalUs[q] * 1000.0));
+    while (std::chrono::steady_clock::now() < due) {
+#if defined(__x86_64__)
+      __builtin_ia32_pause();
+#endif
+    }
+    const int st = td16::serve_submit_at(&reqs[q], 0, &tickets[(size_t)q], due, msg, cap);
+    if (st != TD16_OK) return st;
+  }
+  for (int q = 0; q < count; ++q) {
+    const int st = td16_serve_wait(tickets[(size_t)q], &results[q], msg, cap);
+    if (st != TD16_OK) return st;
+  }
+  if (elapsedUs)
+    *elapsedUs = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - start)
+                     .count();
   return TD16_OK;
 }
 

@@ -170,6 +170,27 @@ def serve_stats(reset_latency: bool = False) -> ServeStats:
     return st
 
 
+def serve_replay(reqs: Sequence[Request], arrival_us: Sequence[float]):
+    """Open-loop load: request q arrives arrival_us[q] after the start.
+    Returns (results, elapsed_us, stats) with p50/p99 from scheduled arrival."""
+    n = len(reqs)
+    arr = (Request * max(1, n))(*reqs)
+    arrivals = (ctypes.c_double * max(1, n))(*arrival_us)
+    out = (Result * max(1, n))()
+    el = ctypes.c_double(0)
+    m = msgbuf()
+    check(lib().td16_serve_replay(arr, arrivals, n, out, ctypes.byref(el), m, len(m)), m)
+    return out, el.value, serve_stats()
+
+
+def probe_valu_peak(device: int = 0, iterations: int = 20000) -> float:
+    """Sustained int32 VALU rate (10^12 lane-ops/s)."""
+    v = ctypes.c_double(0)
+    m = msgbuf()
+    check(lib().td16_probe_valu_peak(device, iterations, ctypes.byref(v), m, len(m)), m)
+    return v.value
+
+
 def serve_stop() -> None:
     m = msgbuf()
     check(lib().td16_serve_stop(m, len(m)), m)

@@ -1,0 +1,195 @@
+"""CPU tests of libtd16.so's host side and of the Python mirrors of the
+reference's dispatcher classes.  No GPU call is made here."""
+import json
+import os
+import random
+import re
+
+import numpy as np
+import pytest
+
+from td16_amd import GpuDeviceLayout, GpuSearchStats, Td16Error
+from td16_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def test_library_exports_every_declared_symbol(native):
+    with open(os.path.join(ROOT, "include", "td16_abi.h")) as f:
+        header = f.read()
+    declared = set(re.findall(r"\b(td16_[a-z0-9_]+)\s*\(", header))
+    assert len(declared) >= 25
+    lib = _lib.lib()
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    # and the binding covers all of them
+    assert declared == set(_lib.SIGNATURES)
+
+
+def test_no_fallback_when_library_missing(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(ImportError):
+        _lib.lib()
+
+
+def test_generators_match_oracle(O, native):
+    for seed, n in ((0, 10), (4, 7), (9, 16)):
+        assert np.array_equal(native.gen_uniform(seed, n, 500), O.gen_uniform(seed, n, 500))
+    a = native.gen_products(1, 12, 16, 4, 9, 3000)
+    b = O.gen_products(1, 12, 16, 4, 9, 3000)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    with open(os.path.join(GOLDEN, "instances.json")) as f:
+        inst = json.load(f)
+    assert [int(x) for x in native.gen_uniform(0, 10, 8)] == inst["config1"]["first"]
+    gaps = native.gen_exponential(3, 20000)
+    assert abs(gaps.mean() - 1.0) < 0.03 and gaps.min() >= 0.0
+
+
+def test_linear_request_matches_oracle_pair(O, native):
+    ns, ks, perms = O.gen_products(8, 3, 16, 1, 9, 200)
+    for q in range(200):
+        req = native.linear_request(int(perms[q]), int(ns[q]), int(ks[q]))
+        beta, omega = O.linear_to_pair(perms[q], int(ns[q]))
+        assert list(req.beta[:req.n]) == beta and list(req.omega[:req.n]) == omega
+        assert req.initial_even == O.count_odd_cycles(omega)
+        t = (req.n - req.initial_even) // 2
+        assert req.min_rate == float(np.float32(2) / (np.float32(ks[q]) / np.float32(t)))
+
+
+def test_validator_and_positions_match_oracle(O, native):
+    rng = random.Random(5)
+    ns, ks, perms = O.gen_products(12, 6, 12, 2, 6, 150)
+    for q in range(150):
+        beta, omega = O.linear_to_pair(perms[q], int(ns[q]))
+        r = O.search(beta, omega, O.count_odd_cycles(omega), 0.0, int(ks[q]), True)
+        assert r.found == 1
+        pos = r.witness()
+        sym = native.positions_to_symbols(beta, pos)
+        assert sym == O.positions_to_symbols(beta, pos)
+        native.validate(beta, omega, sym, int(ks[q]))            # valid: no raise
+        assert O.validate(beta, omega, sym, int(ks[q])) == 0
+        # corruptions the reference rejects: too long, permuted triple, truncated
+        with pytest.raises(Td16Error):
+            native.validate(beta, omega, sym, len(sym) - 1)
+        a, b, c = sym[0]
+        bad = [(a, c, b)] + list(sym[1:])
+        with pytest.raises(Td16Error):
+            native.validate(beta, omega, bad, int(ks[q]))
+        assert O.validate(beta, omega, bad, int(ks[q])) != 0
+        if len(sym) > 1:
+            with pytest.raises(Td16Error):
+                native.validate(beta, omega, sym[:-1], int(ks[q]))
+            assert O.validate(beta, omega, sym[:-1], int(ks[q])) != 0
+    with pytest.raises(Td16Error):
+        native.validate([0, 1, 2], [1, 2, 0], [], 1)
+
+
+def test_codec_matches_oracle(O, native):
+    rng = random.Random(2)
+    for _ in range(300):
+        n = rng.randint(1, 17)
+        om = list(range(n))
+        rng.shuffle(om)
+        assert native.codec_key(om) == O.codec_key(om)
+        assert native.codec_decode_key(native.codec_key(om), n) == om
+        m = rng.randint(1, 9)
+        moves = [tuple(rng.sample(range(17), 3)) for _ in range(m)]
+        v = native.codec_value(moves)
+        assert v == O.codec_value(moves) and len(v) == (15 * m + 7) // 8
+        assert native.codec_decode_value(v) == moves
+    # CT/base/CertificateCodecTest.java:14-38
+    assert len(native.codec_key([1, 2, 0])) == 2
+    assert len(native.codec_value([(0, 1, 2)])) == 2
+    with pytest.raises(Td16Error):
+        native.codec_decode_value(b"\x00")
+    with pytest.raises(Td16Error):
+        native.codec_value([])
+    with pytest.raises(Td16Error):
+        native.codec_value([(0, 1, 32)])
+
+
+def test_search_argument_errors_without_gpu(native):
+    # jni_sorter.cu:61-67, sorting_search.cu:820-825: argument errors surface
+    # before any device work
+    with pytest.raises(Td16Error) as e:
+        native.search_positions(0, list(range(18)), list(range(18)), 1, 1.0, 3)
+    assert e.value.status == _lib.INVALID_ARGUMENT
+    with pytest.raises(Td16Error) as e:
+        native.search_positions(0, [0, 1, 2], [1, 2], 1, 1.0, 1)
+    assert e.value.status == _lib.INVALID_ARGUMENT
+    with pytest.raises(Td16Error) as e:
+        native.init_slots(2, [], 1 << 20)
+    assert e.value.status == _lib.INVALID_ARGUMENT
+    with pytest.raises(Td16Error) as e:
+        native.init_slots(0, [0], 1 << 20)
+    assert e.value.status == _lib.INVALID_ARGUMENT
+    with pytest.raises(Td16Error) as e:
+        native.init_slots(1, [0], 0)
+    assert e.value.status == _lib.INVALID_ARGUMENT
+    req = native.make_request([0, 1, 2], [1, 2, 0], 1, 2.0, 1)
+    with pytest.raises(Td16Error) as e:
+        native.search_ex(0, req)
+    assert e.value.status == _lib.NOT_INITIALIZED
+
+
+# --- GpuDeviceLayout (ET/base/GpuDeviceLayoutTest.java:11-44) -------------
+
+def test_layout_parses_and_maps_round_robin():
+    devices = GpuDeviceLayout.parse_devices(" 2, 5 ")
+    layout = GpuDeviceLayout(devices, 3)
+    assert layout.devices() == [2, 5]
+    assert layout.total_slots() == 6
+    assert [layout.device_for_slot(s) for s in (0, 1, 2, 5)] == [2, 5, 2, 5]
+
+
+def test_layout_rejects_duplicate_or_malformed():
+    for bad in (lambda: GpuDeviceLayout([0, 0], 7), lambda: GpuDeviceLayout([-1], 7),
+                lambda: GpuDeviceLayout.parse_devices("0,,1"),
+                lambda: GpuDeviceLayout.parse_devices("gpu0"),
+                lambda: GpuDeviceLayout.parse_devices(" ")):
+        with pytest.raises(ValueError):
+            bad()
+
+
+def test_layout_rejects_invalid_or_overflowing_slot_counts():
+    with pytest.raises(ValueError):
+        GpuDeviceLayout([0], 0)
+    with pytest.raises(ValueError):
+        GpuDeviceLayout([0, 1], 2**31 - 1)
+    with pytest.raises(ValueError):
+        GpuDeviceLayout([0], 2).device_for_slot(2)
+
+
+# --- GpuSearchStats (ET/base/GpuSearchStatsTest.java:9-55) ----------------
+
+def test_stats_attribute_wait_busy_and_completions():
+    stats = GpuSearchStats(GpuDeviceLayout([0, 1], 2), 0)
+    stats.acquired(0, 5, 10)
+    stats.acquired(1, 10, 20)
+    stats.released(0, 60, False)
+    first = stats.snapshot(100)
+    assert (first.started, first.completed, first.failed) == (2, 1, 0)
+    assert first.queue_wait_nanos == 15 and first.busy_nanos == 130
+    assert first.devices[0].completed == 1 and first.devices[0].busy_nanos == 50
+    assert first.devices[1].completed == 0 and first.devices[1].busy_nanos == 80
+    stats.released(1, 180, True)
+    second = stats.snapshot(200)
+    assert (second.started, second.completed, second.failed) == (0, 1, 1)
+    assert second.busy_nanos == 80
+    assert second.devices[0].busy_nanos == 0 and second.devices[1].busy_nanos == 80
+
+
+def test_stats_window_reset_keeps_active_search():
+    stats = GpuSearchStats(GpuDeviceLayout([3], 1), 1000)
+    stats.acquired(0, 1000, 1100)
+    first = stats.snapshot(1500)
+    second = stats.snapshot(1800)
+    stats.released(0, 2000, False)
+    third = stats.snapshot(2100)
+    assert (first.busy_nanos, second.busy_nanos, third.busy_nanos) == (400, 300, 200)
+    assert third.completed == 1 and third.started == 0
+    with pytest.raises(RuntimeError):
+        stats.released(0, 2200, False)

@@ -86,7 +86,13 @@ typedef struct {
  *   solutions: witnesses met (COUNT_ALL: all of them)
  *   split: 1 when the search was shared by several wavefronts (its FIRST
  *     witness and counts may then differ from a sequential DFS's; found /
- *     not-found never does) */
+ *     not-found never does)
+ *   device_us: device time from the request's start on a wavefront to its
+ *     result's publication (the chip's 100 MHz real-time counter)
+ *   latency_us: served loop only, submit (or scheduled arrival) to the
+ *     result's arrival on the host
+ *   queue_us: served loop only, submit (or scheduled arrival) to the
+ *     request's publication to the device (time spent waiting for a slot) */
 typedef struct {
     int32_t status;
     int32_t found;
@@ -95,6 +101,9 @@ typedef struct {
     int32_t moves[TD16_MAX_DEPTH][3];
     uint64_t expanded;
     uint64_t solutions;
+    double device_us;
+    double latency_us;
+    double queue_us;
 } td16_result;
 
 /* ---------------------------------------------------------------------- */
@@ -150,7 +159,7 @@ int td16_search_ex(int slot, const td16_request *req, td16_result *res,
 
 typedef struct {
     int32_t device;          /* HIP device ordinal */
-    int32_t waves_per_cu;    /* 0 = occupancy maximum */
+    int32_t waves_per_cu;    /* resident wavefronts per CU; 0 = 12 (tuned) */
     int32_t split_depth;     /* frames deeper than max_moves - split_depth are
                                 never donated; 0 = default (5) */
     int32_t donate_max;      /* children donated per donation; 0 = default (64) */
@@ -184,7 +193,7 @@ typedef struct {
     int32_t slots;           /* requests admitted on the device at once (7..4096) */
     int32_t queue_capacity;  /* requests waiting for a slot (the reference's
                                 tdp.gpuQueueCapacity, GPUSortingSearch.java:82-89) */
-    int32_t waves_per_cu;    /* 0 = occupancy maximum */
+    int32_t waves_per_cu;    /* resident wavefronts per CU; 0 = 4 (tuned for latency) */
     int64_t slot_bytes;      /* HBM budget per slot (tdp.gpuBudgetMB) */
 } td16_serve_options;
 

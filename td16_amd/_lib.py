@@ -73,6 +73,9 @@ class Result(ctypes.Structure):
         ("moves", (ctypes.c_int32 * 3) * MAX_DEPTH),
         ("expanded", ctypes.c_uint64),
         ("solutions", ctypes.c_uint64),
+        ("device_us", ctypes.c_double),
+        ("latency_us", ctypes.c_double),
+        ("queue_us", ctypes.c_double),
     ]
 
     def witness(self):

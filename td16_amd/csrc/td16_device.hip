@@ -190,8 +190,11 @@ enum { kExpandOk = 0, kExpandFound = 1 };
 // Packs the witness into the job state when this wave is the first finder.
 __device__ void record_witness(const Params &P, Ctx &c, WaveLds &w, int depth, int lastMove) {
   JobState *st = &P.states[c.job];
-  // the witness is published by the release on `pending` at the item's end
-  const int won = lane0([&] { return cas_rlx(&st->found, 0, 1) ? 1 : 0; });
+  // the witness is published by the release on `pending` at the item's end;
+  // a request no other wavefront has touched needs no atomic at all
+  int won = 1;
+  if (c.shared) won = lane0([&] { return cas_rlx(&st->found, 0, 1) ? 1 : 0; });
+  else if (lane_id() == 0) st->found = 1;
   if (!won) return;
   const int lane = lane_id();
   if (lane < depth) {
@@ -607,6 +610,17 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
   }
   // flush counters, then release this item (the release publishes the
   // counters and any witness to the wavefront that finalizes the request)
+  if (!c.shared) {
+    // private request (root item, never donated from): this wave is the
+    // only one that ever held it, so it finalizes with plain stores
+    if (lane == 0) {
+      st->expanded += c.expanded;
+      st->solutions += c.solutions;
+      st->pending = 0;
+    }
+    wave_sync();
+    return true;
+  }
   int last = 0;
   if (lane == 0) {
     if (c.expanded) add_rlx(&st->expanded, c.expanded);
@@ -615,7 +629,7 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
                                   __HIP_MEMORY_SCOPE_AGENT) == 1 ? 1 : 0;
   }
   last = uni(last);
-  if (last && (c.shared || uni(ld_rlx(&st->split)))) fence_acq();
+  if (last) fence_acq();
   return last != 0;
 }
 
@@ -637,6 +651,8 @@ __device__ void publish_result(const Params &P, uint32_t j) {
       r.tag = P.jobs[j].tag;
       r.seq = 0;
       r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0;
+      r.t0 = st->t0;
+      r.t1 = __builtin_amdgcn_s_memrealtime();
       P.results[j] = r;
       add_rlx(&P.ctrl->jobsDone, 1u);
     }
@@ -655,6 +671,8 @@ __device__ void publish_result(const Params &P, uint32_t j) {
       r.tag = P.jobs[j].tag;
       r.seq = 0;
       r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0;
+      r.t0 = st->t0;
+      r.t1 = __builtin_amdgcn_s_memrealtime();
       P.hostResults[ringIdx] = r;
       const uint32_t cslot = (uint32_t)(cseq & (uint64_t)(P.ringCap - 1));
       P.hostCompletionIdx[2 * cslot] = ringIdx;
@@ -679,6 +697,7 @@ __device__ int start_job(const Params &P, uint32_t j) {
     st->expanded = 0;
     st->solutions = 0;
     st->depth = 0;
+    st->t0 = __builtin_amdgcn_s_memrealtime();
   }
   // no fence: until this wave donates (which releases), no other wavefront
   // reads this state
@@ -703,6 +722,8 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut) {
         // stop is read first: the host raises it only after its last submit
         if (ld_sys_acq(&P.host->stop) != 0) st_rlx(&P.ctrl->stopMirror, (uint64_t)1);
         avail = ld_sys_acq(&P.host->submitted);
+        // publish what that acquire made visible before advancing the mirror
+        fence_rel();
         uint64_t cur = ld_rlx(&P.ctrl->submittedMirror);
         while (cur < avail && !cas_rlx(&P.ctrl->submittedMirror, cur, avail))
           cur = ld_rlx(&P.ctrl->submittedMirror);
@@ -720,21 +741,27 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut) {
   got = uni(got);
   if (!got) return 0;
   r = uni64(r);
-  // synchronize with the host's release of `submitted` before reading entry r
-  if (lane == 0) (void)ld_sys_acq(&P.host->submitted);
-  // ring entries are transport only: the record's tag names its job slot
+  // pairs with the poller's release: entry r is visible from here on
+  fence_acq();
+  // one read of the 128-byte record over the bus, 16 bytes per lane; ring
+  // entries are transport only and the record's tag names its job slot
   const uint32_t e = (uint32_t)(r & (uint64_t)(P.ringCap - 1));
   const uint32_t *src = reinterpret_cast<const uint32_t *>(&P.hostJobs[e]);
-  const uint32_t j = (uint32_t)uni((int)__hip_atomic_load(&P.hostJobs[e].tag, __ATOMIC_ACQUIRE,
-                                                         __HIP_MEMORY_SCOPE_SYSTEM));
+  uint32_t v[4] = {0, 0, 0, 0};
+  if (lane < 8) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      v[q] = __hip_atomic_load(src + 4 * lane + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  const uint32_t j = (uint32_t)__builtin_amdgcn_readlane((int)v[3], 7);  // word 31: tag
   if (lane < 8) {
     uint32_t *dst = reinterpret_cast<uint32_t *>(const_cast<Job *>(&P.jobs[j]));
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      dst[4 * lane + q] = __hip_atomic_load(src + 4 * lane + q, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_SYSTEM);
+    for (int q = 0; q < 4; ++q) dst[4 * lane + q] = v[q];
   }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+  // the record is re-read by this wave only (lanes other than its writer):
+  // workgroup scope suffices; a later donation's release publishes it further
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
   jOut = j;
   return 1;
 }

@@ -56,8 +56,10 @@ struct alignas(64) JobState {
   int32_t depth;         // witness length
   uint16_t moves[10];    // witness, packed i | j<<5 | k<<10
   uint8_t pad[4];
+  uint64_t t0;           // device real-time counter (100 MHz) when the request started
+  uint8_t pad2[56];
 };
-static_assert(sizeof(JobState) == 64, "JobState layout");
+static_assert(sizeof(JobState) == 128, "JobState layout");
 
 // What the host reads back per request (64 B).
 struct alignas(16) DevResult {
@@ -71,8 +73,9 @@ struct alignas(16) DevResult {
   uint32_t tag;
   uint32_t seq;          // served loop: completion sequence + 1 once written
   uint8_t pad[4];
+  uint64_t t0, t1;       // device real-time counter at start and at publication
 };
-static_assert(sizeof(DevResult) == 64, "DevResult layout");
+static_assert(sizeof(DevResult) == 80, "DevResult layout");
 
 // A donated subtree: a node still to be expanded (48 B).
 struct alignas(16) Item {

@@ -171,6 +171,7 @@ void decode(const DevResult &r, td16_result *out) {
   out->split = r.split;
   out->expanded = r.expanded;
   out->solutions = r.solutions;
+  out->device_us = (r.t1 >= r.t0) ? (double)(r.t1 - r.t0) * 0.01 : 0.0;  // 100 MHz ticks
   unpack_moves(r.moves, out->depth, out->moves);
 }
 
@@ -272,7 +273,8 @@ void reset_batch_graph() {
 struct Ticket {
   bool ready = false;
   td16_result res;
-  std::chrono::steady_clock::time_point t0;
+  std::chrono::steady_clock::time_point t0;      // arrival
+  std::chrono::steady_clock::time_point tAdmit;  // published to the device
 };
 
 struct Server {
@@ -292,7 +294,7 @@ struct Server {
   uint32_t *dComp = nullptr;
 
   std::mutex mu;
-  std::condition_variable cvAdmit, cvDone, cvSpace;
+  std::condition_variable cvDone, cvSpace, cvWork;
   std::deque<std::pair<uint64_t, Job>> waiting;
   std::vector<uint32_t> freeSlots;
   std::vector<uint64_t> slotTicket;
@@ -301,7 +303,7 @@ struct Server {
   uint64_t submittedSeq = 0;
   uint32_t inFlight = 0;
   bool stopping = false;
-  std::thread admitter, poller;
+  std::thread poller;
   std::atomic<bool> pollerExit{false};
 
   uint64_t nSubmitted = 0, nCompleted = 0, nRejected = 0;
@@ -311,39 +313,48 @@ struct Server {
 std::mutex gServeMu;  // start/stop
 Server *gServer = nullptr;
 
-void admit_loop(Server *S) {
-  std::unique_lock<std::mutex> lk(S->mu);
-  for (;;) {
-    S->cvAdmit.wait(lk, [&] {
-      return (!S->waiting.empty() && S->inFlight < S->slots) ||
-             (S->stopping && S->waiting.empty());
-    });
-    if (S->stopping && S->waiting.empty()) return;
+// Publishes waiting requests while slots are free (caller holds S->mu).
+// Called by submit and by the completion poller, so no thread hop sits
+// between a free slot and the next request.
+void admit_locked(Server *S) {
+  bool any = false;
+  while (!S->waiting.empty() && S->inFlight < S->slots) {
     auto item = S->waiting.front();
     S->waiting.pop_front();
-    S->cvSpace.notify_all();
     const uint32_t slot = S->freeSlots.back();
     S->freeSlots.pop_back();
     S->slotTicket[slot] = item.first;
-    S->inFlight++;
+    auto tk = S->tickets.find(item.first);
+    if (tk != S->tickets.end()) tk->second.tAdmit = std::chrono::steady_clock::now();
+    __atomic_fetch_add(&S->inFlight, 1u, __ATOMIC_RELAXED);
     const uint64_t r = S->submittedSeq++;
     Job job = item.second;
     job.tag = slot;
     std::memcpy(&S->hRing[r & (S->ringCap - 1)], &job, sizeof(Job));
+    any = true;
+  }
+  if (any) {
     std::atomic_thread_fence(std::memory_order_release);
-    __atomic_store_n(&S->hHost->submitted, r + 1, __ATOMIC_RELEASE);
+    __atomic_store_n(&S->hHost->submitted, S->submittedSeq, __ATOMIC_RELEASE);
+    S->cvSpace.notify_all();
+    S->cvWork.notify_one();  // the poller may be parked on an idle loop
   }
 }
 
+// Completion poller: spins on the next completion record while requests are
+// in flight (a dedicated core buys microseconds of latency); with nothing in
+// flight it parks on cvWork, which admission signals.
 void poll_loop(Server *S) {
   uint64_t c = 0;
-  int idleSpins = 0;
   while (!S->pollerExit.load(std::memory_order_acquire)) {
     const uint32_t e = (uint32_t)(c & (S->ringCap - 1));
     const uint32_t seq = __atomic_load_n(&S->hComp[2 * e + 1], __ATOMIC_ACQUIRE);
     if (seq != (uint32_t)(c + 1)) {
-      if (++idleSpins > 2000) {
-        std::this_thread::sleep_for(std::chrono::microseconds(20));
+      if (__atomic_load_n(&S->inFlight, __ATOMIC_RELAXED) == 0) {
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->cvWork.wait_for(lk, std::chrono::milliseconds(1), [&] {
+          return S->inFlight > 0 || S->pollerExit.load(std::memory_order_acquire);
+        });
       } else {
 #if defined(__x86_64__)
         __builtin_ia32_pause();
@@ -351,7 +362,6 @@ void poll_loop(Server *S) {
       }
       continue;
     }
-    idleSpins = 0;
     const uint32_t slot = __atomic_load_n(&S->hComp[2 * e], __ATOMIC_ACQUIRE);
     DevResult r;
     std::memcpy(&r, &S->hRes[slot], sizeof(DevResult));
@@ -363,15 +373,18 @@ void poll_loop(Server *S) {
       if (it != S->tickets.end()) {
         decode(r, &it->second.res);
         it->second.ready = true;
-        S->latUs.push_back(
-            std::chrono::duration<double, std::micro>(now - it->second.t0).count());
+        const double us = std::chrono::duration<double, std::micro>(now - it->second.t0).count();
+        it->second.res.latency_us = us;
+        it->second.res.queue_us =
+            std::chrono::duration<double, std::micro>(it->second.tAdmit - it->second.t0).count();
+        S->latUs.push_back(us);
       }
       S->freeSlots.push_back(slot);
-      S->inFlight--;
+      __atomic_fetch_sub(&S->inFlight, 1u, __ATOMIC_RELAXED);
       S->nCompleted++;
+      admit_locked(S);
     }
     S->cvDone.notify_all();
-    S->cvAdmit.notify_all();
     ++c;
   }
 }
@@ -596,8 +609,10 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
     if (st != TD16_OK) return st;
   }
   Buffers &b = gBatch.buf;
-  int perCU = dev.blocksPerCU;
-  if (o.waves_per_cu > 0) perCU = std::max(1, std::min(perCU, o.waves_per_cu / kWavesPerBlock));
+  // 0 = the measured best for the batched mix: 12 waves per CU (3 per SIMD);
+  // more resident waves only add idle polling once the feed drains
+  const int wantWaves = o.waves_per_cu > 0 ? o.waves_per_cu : 12;
+  const int perCU = std::max(1, std::min(dev.blocksPerCU, wantWaves / kWavesPerBlock));
   const int blocks = dev.cus * perCU;
   localStats.blocks = blocks;
   localStats.waves = blocks * kWavesPerBlock;
@@ -733,12 +748,12 @@ int td16_serve_start(const td16_serve_options *opt, char *msg, int cap) {
   P.hostJobs = S->dRing;
   P.hostResults = S->dRes;
   P.hostCompletionIdx = S->dComp;
-  int perCU = dev.blocksPerCU;
-  if (opt->waves_per_cu > 0) perCU = std::max(1, std::min(perCU, opt->waves_per_cu / kWavesPerBlock));
+  // 0 = the measured best for latency: 4 waves per CU (one per SIMD)
+  const int wantWaves = opt->waves_per_cu > 0 ? opt->waves_per_cu : 4;
+  const int perCU = std::max(1, std::min(dev.blocksPerCU, wantWaves / kWavesPerBlock));
   TD16_HIP((hipError_t)launch_search(P, dev.cus * perCU, S->stream));
   S->running = true;
   Server *raw = S.release();
-  raw->admitter = std::thread(admit_loop, raw);
   raw->poller = std::thread(poll_loop, raw);
   gServer = raw;
   return TD16_OK;
@@ -794,7 +809,7 @@ int serve_submit_at(const td16_request *req, int nonblocking, uint64_t *ticket,
     S->cvSpace.wait(lk, [&] { return S->waiting.size() < limit || S->stopping; });
   }
   S->waiting.emplace_back(t, job);
-  S->cvAdmit.notify_one();
+  admit_locked(S);
   return TD16_OK;
 }
 }  // namespace
@@ -883,11 +898,9 @@ int td16_serve_stop(char *msg, int cap) {
   {
     std::unique_lock<std::mutex> l2(S->mu);
     S->stopping = true;
-    S->cvAdmit.notify_all();
     S->cvSpace.notify_all();
     S->cvDone.wait(l2, [&] { return S->waiting.empty() && S->inFlight == 0; });
   }
-  S->admitter.join();
   __atomic_store_n(&S->hHost->stop, 1u, __ATOMIC_RELEASE);
   hipSetDevice(S->device);
   hipError_t e = hipStreamSynchronize(S->stream);

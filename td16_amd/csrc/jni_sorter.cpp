@@ -15,14 +15,27 @@
 //    and released with JNI_ABORT (:71-83);
 //  - every failure becomes java.lang.RuntimeException(message) and the
 //    method returns null (:15-22, 111-116).
+//
+// With TD16_SERVE=1 in the environment, initSlots starts the served loop on
+// the first device instead (slots = totalSlots, queue capacity from
+// TD16_SERVE_QUEUE, default 512) and every searchForSortingSeq becomes a
+// ticket in it: the Java side is unchanged, its slot threads become
+// producers for one resident kernel.
 #include <jni.h>
 
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "../../include/td16_abi.h"
 
 namespace {
+
+bool served_mode() {
+  const char *v = std::getenv("TD16_SERVE");
+  return v != nullptr && std::strcmp(v, "1") == 0;
+}
 
 void throw_runtime(JNIEnv *env, const char *message) {
   if (env->ExceptionCheck()) env->ExceptionClear();
@@ -58,9 +71,25 @@ extern "C" JNIEXPORT void JNICALL Java_br_unb_cic_tdp_base_GPUSortingSearch_init
   std::vector<jint> tmp(static_cast<size_t>(nd > 0 ? nd : 1));
   if (nd > 0) env->GetIntArrayRegion(javaDevices, 0, nd, tmp.data());
   std::vector<int32_t> devices(tmp.begin(), tmp.begin() + nd);
-  const int st = td16_init_slots(static_cast<int>(totalSlots), devices.data(),
-                                 static_cast<int>(nd), static_cast<int64_t>(queueBytesBudget),
-                                 msg, static_cast<int>(sizeof(msg)));
+  int st;
+  if (served_mode()) {
+    if (nd < 1) {
+      throw_runtime(env, "devices must contain at least one device ordinal");
+      return;
+    }
+    td16_serve_options o;
+    std::memset(&o, 0, sizeof(o));
+    o.device = devices[0];
+    o.slots = static_cast<int32_t>(totalSlots);
+    const char *q = std::getenv("TD16_SERVE_QUEUE");
+    o.queue_capacity = q ? std::atoi(q) : 512;
+    o.slot_bytes = static_cast<int64_t>(queueBytesBudget);
+    st = td16_serve_start(&o, msg, static_cast<int>(sizeof(msg)));
+  } else {
+    st = td16_init_slots(static_cast<int>(totalSlots), devices.data(), static_cast<int>(nd),
+                         static_cast<int64_t>(queueBytesBudget), msg,
+                         static_cast<int>(sizeof(msg)));
+  }
   if (st != TD16_OK) throw_runtime(env, msg);
 }
 
@@ -93,10 +122,32 @@ Java_br_unb_cic_tdp_base_GPUSortingSearch_searchForSortingSeq(
   }
   int32_t positions[TD16_MAX_DEPTH * 3];
   int32_t len = 0;
-  const int st = td16_search(static_cast<int>(slot), beta, omega, static_cast<int>(n),
-                             static_cast<int>(initialEvenCycles), static_cast<float>(minRate),
-                             static_cast<int>(maxMoves), fullSorting ? 1 : 0, positions, &len,
-                             msg, static_cast<int>(sizeof(msg)));
+  int st;
+  if (served_mode()) {
+    td16_request req;
+    std::memset(&req, 0, sizeof(req));
+    std::memcpy(req.beta, beta, static_cast<size_t>(n));
+    std::memcpy(req.omega, omega, static_cast<size_t>(n));
+    req.n = static_cast<int32_t>(n);
+    req.initial_even = static_cast<int32_t>(initialEvenCycles);
+    req.min_rate = static_cast<float>(minRate);
+    req.max_moves = static_cast<int32_t>(maxMoves);
+    req.full_sorting = fullSorting ? 1 : 0;
+    req.moveset = TD16_MOVES_REF;
+    req.mode = TD16_FIRST;
+    td16_result res;
+    st = td16_serve_search(&req, &res, msg, static_cast<int>(sizeof(msg)));
+    if (st == TD16_OK) {
+      len = res.found ? res.depth : 0;
+      for (int d = 0; d < len; ++d)
+        for (int q = 0; q < 3; ++q) positions[3 * d + q] = res.moves[d][q];
+    }
+  } else {
+    st = td16_search(static_cast<int>(slot), beta, omega, static_cast<int>(n),
+                     static_cast<int>(initialEvenCycles), static_cast<float>(minRate),
+                     static_cast<int>(maxMoves), fullSorting ? 1 : 0, positions, &len, msg,
+                     static_cast<int>(sizeof(msg)));
+  }
   if (st != TD16_OK) {
     throw_runtime(env, msg);
     return nullptr;

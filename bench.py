@@ -167,7 +167,15 @@ def main() -> int:
     ap.add_argument("--served-slots", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--skip", default="", help="comma list of: plain,served,cpu,probe")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (twice) for counter "
+                         "calibration and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        from td16_amd import native
+        gbps = native.probe_copy(0, args.copy_calibration << 20)
+        print(json.dumps({"copy_mib": args.copy_calibration, "read_plus_write_gbps": gbps}))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs each search on one device and "
               "takes no multi-GPU figure (--gpus must be 1)", file=sys.stderr)

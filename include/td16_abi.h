@@ -286,6 +286,11 @@ int td16_linear_request(uint64_t perm, int n, int k, int moveset, int mode, int 
  * search kernel's integer-op roofline (the search issues no MFMA). */
 int td16_probe_valu_peak(int device, int iterations, double *tops, char *msg, int msg_cap);
 
+/* Copies `bytes` bytes between two device buffers with a plain grid-stride
+ * kernel (launched twice; the second launch is timed) and reports read+write
+ * GB/s: a kernel of known HBM traffic for calibrating byte counters. */
+int td16_probe_copy(int device, int64_t bytes, double *gbps, char *msg, int msg_cap);
+
 /* Library version string. */
 const char *td16_version(void);
 

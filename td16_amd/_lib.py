@@ -153,6 +153,7 @@ SIGNATURES = {
     "td16_serve_replay": (_i, [ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double), _i,
                                ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_double), _c, _i]),
     "td16_probe_valu_peak": (_i, [_i, _i, ctypes.POINTER(ctypes.c_double), _c, _i]),
+    "td16_probe_copy": (_i, [_i, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), _c, _i]),
     "td16_positions_to_symbols": (_i, [_u8p, _i, _i32p, _i, _i32p, _c, _i]),
     "td16_validate": (_i, [_u8p, _u8p, _i, _i32p, _i, _i, _c, _i]),
     "td16_codec_key": (_i, [_u8p, _i, _u8p, _i, _i32p, _c, _i]),

@@ -70,3 +70,48 @@ extern "C" int td16_probe_valu_peak(int device, int iterations, double *tops, ch
   *tops = ops / (ms * 1e-3) / 1e12;
   return TD16_OK;
 }
+
+namespace td16 {
+namespace {
+// Grid-stride 16-byte copy: a kernel of known HBM traffic (bytes read and
+// written) against which hardware byte counters are calibrated.
+__global__ void __launch_bounds__(256) copy_probe_kernel(const uint4 *src, uint4 *dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+}  // namespace td16
+
+extern "C" int td16_probe_copy(int device, int64_t bytes, double *gbps, char *msg, int cap) {
+  using namespace td16;
+  if (gbps == nullptr || bytes < 16)
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid probe arguments");
+  if (hipSetDevice(device) != hipSuccess)
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid device %d", device);
+  const uint64_t n = (uint64_t)bytes / 16;
+  uint4 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return set_msg(msg, cap, TD16_OUT_OF_MEMORY, "probe allocation failed");
+  }
+  (void)hipMemset(a, 1, n * 16);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const unsigned blocks = 256 * 16;
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+  (void)hipEventRecord(e0, nullptr);
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+  (void)hipEventRecord(e1, nullptr);
+  const hipError_t e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return set_msg(msg, cap, TD16_DEVICE_ERROR, "probe failed");
+  *gbps = 2.0 * (double)(n * 16) / (ms * 1e-3) / 1e9;
+  return TD16_OK;
+}

@@ -191,6 +191,14 @@ def probe_valu_peak(device: int = 0, iterations: int = 20000) -> float:
     return v.value
 
 
+def probe_copy(device: int = 0, nbytes: int = 256 << 20) -> float:
+    """Read+write GB/s of a plain device copy of nbytes."""
+    v = ctypes.c_double(0)
+    m = msgbuf()
+    check(lib().td16_probe_copy(device, int(nbytes), ctypes.byref(v), m, len(m)), m)
+    return v.value
+
+
 def serve_stop() -> None:
     m = msgbuf()
     check(lib().td16_serve_stop(m, len(m)), m)

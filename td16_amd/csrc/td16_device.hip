@@ -4,8 +4,8 @@
 // One wavefront is one DFS worker.  A node is expanded by the whole wave:
 //   1. set-up: lanes 0..N-1 own one sequence position each; the cycle
 //      structure of omega (cycle, length, distance to the cycle's smallest
-//      position) is found by 2 x 4 rounds of pointer jumping over
-//      ds_bpermute, no LDS tables and no serial walks;
+//      position) is found by 4 rounds of pointer doubling over ds_bpermute,
+//      no LDS tables and no serial walks;
 //   2. candidates: the C(m,3) position triples (m = breakpoints under the
 //      reference filter, all N positions otherwise) are evaluated 64 lanes at
 //      a time in lexicographic order; each lane gets the odd-cycle delta of
@@ -20,6 +20,8 @@
 // :240-332; witness and prune predicates :336-373 (host-tabulated); positions
 // returned exactly as :956-971 decodes them.
 #include <hip/hip_runtime.h>
+
+#include <cstddef>
 
 #include "td16_internal.h"
 
@@ -181,8 +183,9 @@ struct Ctx {
   uint64_t expanded;
   uint64_t solutions;
   int sinceCheck;
-  int home;        // this wave's queue shard
+  int home;        // this wave's queue shard and feed shard
   uint32_t rng;    // LCG state for random shard probes
+  bool drained;    // batched: this wave saw every feed shard exhausted
 };
 
 enum { kExpandOk = 0, kExpandFound = 1 };
@@ -232,27 +235,26 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   nx = inN ? nx : p;  // lanes past N: harmless fixed points
   (void)sh;
 
-  // cycle members within distance 16: 4 doubling rounds
+  // Cycle of position p under nx by pointer doubling.  Before round r the
+  // window is nx^i(p) for 0 <= i <= 2^r: M holds its members, J = nx^(2^r)(p)
+  // and D the first i at which the window's smallest member (the cycle's
+  // leader once the window covers the cycle) occurs.  One bpermute per round
+  // carries J | M << 5 | D << 22; four rounds cover cycles of up to 17.
   int J = nx & 31;
   uint32_t M = (1u << (p & 31)) | (1u << J);
+  int D = J < p ? 1 : 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int got = bperm(J, (int)(((uint32_t)J) | (M << 5)));
-    M |= ((uint32_t)got) >> 5;
-    J = got & 31;
+    const uint32_t got =
+        (uint32_t)bperm(J, (int)((uint32_t)J | ((M & 0x1FFFFu) << 5) | ((uint32_t)D << 22)));
+    const uint32_t Mj = (got >> 5) & 0x1FFFFu;
+    if (__builtin_ctz(Mj) < __builtin_ctz(M)) D = (1 << r) + (int)(got >> 22);
+    M |= Mj;
+    J = (int)(got & 31u);
   }
   const int leader = __builtin_ctz(M);
   const int L = __builtin_popcount(M);
-  // forward distance to the leader: 4 doubling rounds with the leader absorbing
   const bool isLeader = (p == leader);
-  int J2 = isLeader ? p : (nx & 31);
-  int D = isLeader ? 0 : 1;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int got = bperm(J2, J2 | (D << 5));
-    D += got >> 5;
-    J2 = got & 31;
-  }
   const int ec = __builtin_popcountll(ballot(inN && isLeader && (L & 1)));
   const uint64_t bpMask = ballot(inN && nx != p);
   const int info = D | (L << 8) | (leader << 16);
@@ -449,9 +451,8 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   // the release fence below orders these before any consumer's claim
   if (lane == 0) {
     add_rlx(&P.states[c.job].pending, cnt);
-    st_rlx(&P.states[c.job].split, 1);
-    add_rlx(&P.ctrl->donations, (uint64_t)cnt);
-    add_rlx(&P.ctrl->queued, (int64_t)cnt);
+    add_rlx(&P.states[c.job].split, cnt);
+    add_rlx(&P.ctrl->balance[c.home % kBalanceShards].v, (int64_t)cnt);
   }
   const uint64_t slot = (uint64_t)shard * cap + pos % cap;
   if (ok) {
@@ -502,7 +503,6 @@ __device__ int pop_item(const Params &P, Ctx &c, Item &it, WaveLds &w) {
   const uint64_t slot = (uint64_t)shard * cap + h % cap;
   it = P.items[slot];
   if (lane < 10) w.path[lane] = P.items[slot].moves[lane];
-  if (lane == 0) add_rlx(&P.ctrl->queued, (int64_t)-1);
   // the item is in registers once every load has returned: free the slot
   __builtin_amdgcn_s_waitcnt(0);
   if (lane == 0) st_rlx(&P.seq[slot], h + cap);
@@ -513,6 +513,53 @@ __device__ int pop_item(const Params &P, Ctx &c, Item &it, WaveLds &w) {
 // ---------------------------------------------------------------------------
 // Requests
 // ---------------------------------------------------------------------------
+
+// Batched feed: claims the next unstarted request, from the home feed shard
+// first, then from whichever other shard still has some (every lane inspects
+// one shard's cursor).  Returns -1 once every shard is exhausted.
+__device__ int claim_job(const Params &P, Ctx &c) {
+  if (c.drained) return -1;
+  Control *ctl = P.ctrl;
+  const int lane = lane_id();
+  const uint32_t per = ctl->perShard;  // host-written before the launch
+  const uint32_t nj = ctl->numJobs;
+  auto shard_end = [&](uint32_t s) -> uint32_t {
+    const uint64_t e = (uint64_t)(s + 1) * per;
+    return e < nj ? (uint32_t)e : nj;
+  };
+  auto take = [&](int s) {
+    return lane0([&] {
+      const uint32_t end = shard_end((uint32_t)s);
+      if ((uint64_t)s * per >= nj || ld_rlx(&ctl->feed[s].next) >= end) return -1;
+      const uint32_t g = add_rlx(&ctl->feed[s].next, 1u);
+      return g < end ? (int)g : -1;
+    });
+  };
+  int got = take(c.home);
+  if (got >= 0) return got;
+  if (lane0([&] { return (int)ld_rlx(&ctl->drained); }) != 0) {
+    c.drained = true;
+    return -1;
+  }
+  uint64_t tried = 1ULL << c.home;
+  const int r = (c.home + 1) & (kFeedShards - 1);
+  for (;;) {
+    // shards without requests are never read: the host does not reset them
+    const bool has = (uint64_t)lane * per < nj && ld_rlx(&ctl->feed[lane].next) < shard_end((uint32_t)lane);
+    const uint64_t avail = ballot(has) & ~tried;
+    if (avail == 0) {
+      c.drained = true;
+      if (lane == 0 && ld_rlx(&ctl->drained) == 0u) st_rlx(&ctl->drained, 1u);
+      return -1;
+    }
+    // first available shard after the home shard, cyclically
+    const uint64_t rot = r ? ((avail >> r) | (avail << (64 - r))) : avail;
+    const int pick = (__builtin_ctzll(rot) + r) & (kFeedShards - 1);
+    got = take(pick);
+    if (got >= 0) return got;
+    tried |= 1ULL << pick;
+  }
+}
 
 // Loads the request's parameters into the wave context and LDS tables.
 __device__ void load_job(const Params &P, Ctx &c, WaveLds &w, uint32_t j, int limit) {
@@ -589,16 +636,17 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
     if (++c.sinceCheck >= 32) {
       c.sinceCheck = 0;
       const int want = lane0([&] {
-        const int idle = ld_rlx(&P.ctrl->idle);
-        if (idle <= 0) return 0;
+        // balance = queued items - idle wavefronts: donate only what idle
+        // wavefronts are not already being offered
+        int64_t bal = 0;
+#pragma unroll
+        for (int q = 0; q < kBalanceShards; ++q) bal += ld_rlx(&P.ctrl->balance[q].v);
+        if (bal >= 0) return 0;
         const bool drained = P.served
                                  ? ld_rlx(&P.ctrl->consumed) >= ld_rlx(&P.ctrl->submittedMirror)
-                                 : ld_rlx(&P.ctrl->nextJob) >= P.ctrl->numJobs;
+                                 : ld_rlx(&P.ctrl->drained) != 0u;
         if (!drained) return 0;
-        // donate only what idle wavefronts are not already being offered
-        const int64_t demand = (int64_t)idle - ld_rlx(&P.ctrl->queued);
-        if (demand <= 0) return 0;
-        return demand < P.donateMax ? (int)demand : P.donateMax;
+        return -bal < (int64_t)P.donateMax ? (int)-bal : P.donateMax;
       });
       if (want > 0) donate(P, c, w, lex, d, want);
       if (c.shared && !countAll) {
@@ -633,54 +681,62 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
   return last != 0;
 }
 
+// Word q of the 80-byte DevResult record, written by lane q (q < 20) from the
+// request state; the caller has already acquired what other wavefronts wrote.
+__device__ __forceinline__ uint32_t result_word(const Params &P, uint32_t j, int q) {
+  const uint32_t *st = reinterpret_cast<const uint32_t *>(&P.states[j]);
+  // found depth split limit | expanded solutions | moves[10] | tag seq pad | t0 t1
+  int src = -1;
+  if (q < 4) src = (int)((0x02030801u >> (8 * q)) & 0xffu);
+  else if (q < 8) src = q;                    // expanded, solutions
+  else if (q < 13) src = q + 1;               // moves
+  else if (q == 16 || q == 17) src = q;       // t0
+  if (src >= 0) return st[src];
+  if (q == 13) return P.jobs[j].tag;
+  if (q == 18 || q == 19) {
+    const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
+    return q == 18 ? (uint32_t)t1 : (uint32_t)(t1 >> 32);
+  }
+  return 0u;  // seq, pad
+}
+static_assert(offsetof(JobState, found) == 4 && offsetof(JobState, limit) == 8 &&
+                  offsetof(JobState, split) == 12 && offsetof(JobState, expanded) == 16 &&
+                  offsetof(JobState, depth) == 32 && offsetof(JobState, moves) == 36 &&
+                  offsetof(JobState, t0) == 64,
+              "result_word reads JobState by word");
+static_assert(offsetof(DevResult, moves) == 32 && offsetof(DevResult, tag) == 52 &&
+                  offsetof(DevResult, t0) == 64,
+              "result_word writes DevResult by word");
+
 __device__ void publish_result(const Params &P, uint32_t j) {
   const int lane = lane_id();
-  JobState *st = &P.states[j];
-  // the caller already acquired what other wavefronts wrote; the host reads
-  // batched results after the kernel has ended
   if (!P.served) {
+    // the host reads batched results after the kernel has ended
+    if (lane < 20) reinterpret_cast<uint32_t *>(&P.results[j])[lane] = result_word(P, j, lane);
     if (lane == 0) {
-      DevResult r;
-      r.found = st->found;
-      r.depth = st->depth;
-      r.split = st->split;
-      r.limit = st->limit;
-      r.expanded = st->expanded;
-      r.solutions = st->solutions;
-      for (int q = 0; q < 10; ++q) r.moves[q] = st->moves[q];
-      r.tag = P.jobs[j].tag;
-      r.seq = 0;
-      r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0;
-      r.t0 = st->t0;
-      r.t1 = __builtin_amdgcn_s_memrealtime();
-      P.results[j] = r;
-      add_rlx(&P.ctrl->jobsDone, 1u);
+      // feed shard of request j; its last publisher counts the shard done
+      Control *ctl = P.ctrl;
+      const uint32_t per = ctl->perShard;
+      const uint32_t s = j / per;
+      const uint32_t lo = s * per;
+      const uint32_t hi = ctl->numJobs - lo < per ? ctl->numJobs : lo + per;
+      if (add_rlx(&ctl->feed[s].done, 1u) + 1u == hi - lo) add_rlx(&ctl->shardsDone, 1u);
     }
   } else {
+    // record into the host ring slot of job slot j, then its index into the
+    // completion ring; the system fence orders the record before the index
+    if (lane < 20) reinterpret_cast<uint32_t *>(&P.hostResults[j])[lane] = result_word(P, j, lane);
+    __threadfence_system();
     if (lane == 0) {
       const uint64_t cseq = add_rlx(&P.ctrl->completions, (uint64_t)1);
-      const uint32_t ringIdx = j;  // job slot
-      DevResult r;
-      r.found = st->found;
-      r.depth = st->depth;
-      r.split = st->split;
-      r.limit = st->limit;
-      r.expanded = st->expanded;
-      r.solutions = st->solutions;
-      for (int q = 0; q < 10; ++q) r.moves[q] = st->moves[q];
-      r.tag = P.jobs[j].tag;
-      r.seq = 0;
-      r.pad[0] = r.pad[1] = r.pad[2] = r.pad[3] = 0;
-      r.t0 = st->t0;
-      r.t1 = __builtin_amdgcn_s_memrealtime();
-      P.hostResults[ringIdx] = r;
       const uint32_t cslot = (uint32_t)(cseq & (uint64_t)(P.ringCap - 1));
-      P.hostCompletionIdx[2 * cslot] = ringIdx;
+      P.hostCompletionIdx[2 * cslot] = j;
       __threadfence_system();
       st_sys_rel(&P.hostCompletionIdx[2 * cslot + 1], (uint32_t)(cseq + 1));
       add_rlx(&P.ctrl->jobsDone, 1u);
     }
   }
+  wave_sync();
 }
 
 // Starts request j (root item) in this wave; returns its first depth limit.
@@ -783,6 +839,7 @@ td16_search_kernel(Params P) {
   c.shared = false;
   c.home = (int)((blockIdx.x * kWavesPerBlock + wid) % kQueueShards);
   c.rng = 0x9E3779B9u * (blockIdx.x * kWavesPerBlock + wid + 1);
+  c.drained = false;
   bool idle = false;
   int backoff = 1;
   for (;;) {
@@ -792,12 +849,7 @@ td16_search_kernel(Params P) {
     bool haveJob = false;
     if (!haveItem) {
       if (!P.served) {
-        const int jj = lane0([&] {
-          const uint32_t nj = ld_rlx(&P.ctrl->nextJob);
-          if (nj >= P.ctrl->numJobs) return -1;
-          const uint32_t got = add_rlx(&P.ctrl->nextJob, 1u);
-          return got < P.ctrl->numJobs ? (int)got : -1;
-        });
+        const int jj = claim_job(P, c);
         if (jj >= 0) {
           j = (uint32_t)jj;
           haveJob = true;
@@ -808,12 +860,12 @@ td16_search_kernel(Params P) {
     }
     if (!haveItem && !haveJob) {
       if (!idle) {
-        if (lane == 0) add_rlx(&P.ctrl->idle, 1);
+        if (lane == 0) add_rlx(&P.ctrl->balance[c.home % kBalanceShards].v, (int64_t)-1);
         idle = true;
       }
       int done = 0;
       if (!P.served) {
-        done = lane0([&] { return ld_rlx(&P.ctrl->jobsDone) >= P.ctrl->numJobs ? 1 : 0; });
+        done = lane0([&] { return ld_rlx(&P.ctrl->shardsDone) >= (uint32_t)kFeedShards ? 1 : 0; });
       } else {
         // device-side mirrors only: host memory is read by the rate-limited
         // poll in take_submitted, never by every idle wavefront
@@ -833,10 +885,11 @@ td16_search_kernel(Params P) {
       backoff = backoff < maxBackoff ? backoff * 2 : maxBackoff;
       continue;
     }
-    if (idle) {
-      if (lane == 0) add_rlx(&P.ctrl->idle, -1);
-      idle = false;
-    }
+    // balance bookkeeping: an idle wave that takes an item leaves it
+    // unchanged (one item fewer queued, one wave fewer idle)
+    if (haveItem != idle && lane == 0)
+      add_rlx(&P.ctrl->balance[c.home % kBalanceShards].v, haveItem ? (int64_t)-1 : (int64_t)1);
+    idle = false;
     backoff = 1;
 
     uint64_t pi;
@@ -879,7 +932,6 @@ td16_search_kernel(Params P) {
       c.shared = false;
     }
   }
-  if (idle && lane == 0) add_rlx(&P.ctrl->idle, -1);
 }
 
 // Every shard's slot q starts out free for producer position q.

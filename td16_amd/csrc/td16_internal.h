@@ -13,6 +13,7 @@
 //   (emitter/src/native/backtracking/sorting_search.cu:542-561, 899-913).
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 
 namespace td16 {
@@ -50,7 +51,7 @@ struct alignas(64) JobState {
   int32_t pending;       // work items of this request not yet finished
   int32_t found;         // 0/1, set once by the wavefront that records the witness
   int32_t limit;         // current depth limit
-  int32_t split;         // 1 once a subtree was donated
+  int32_t split;         // subtrees donated from this request
   uint64_t expanded;
   uint64_t solutions;
   int32_t depth;         // witness length
@@ -61,7 +62,7 @@ struct alignas(64) JobState {
 };
 static_assert(sizeof(JobState) == 128, "JobState layout");
 
-// What the host reads back per request (64 B).
+// What the host reads back per request (80 B).
 struct alignas(16) DevResult {
   int32_t found;
   int32_t depth;
@@ -100,27 +101,59 @@ struct alignas(128) QueueState {
   uint8_t pad1[120];
 };
 
-// Per-launch control block (reset by the host before each launch).
-struct alignas(128) Control {
-  uint32_t nextJob;      // batched feed cursor
+// Device-scope atomics on one address serialize at the memory side (about
+// 12 ns each on MI355X) while atomics on different 256-byte lines proceed in
+// parallel, so every counter that many wavefronts update is either sharded
+// over such lines or updated rarely.
+
+// Batched feed: the request table is cut into kFeedShards contiguous feed
+// shards of perShard requests; shard s owns [s*perShard, min(numJobs,
+// (s+1)*perShard)).  A wavefront claims from its home shard and scans the
+// others only once that one is exhausted.
+constexpr int kFeedShards = 64;
+static_assert(kFeedShards == kQueueShards, "a wave's home shard indexes both");
+struct alignas(256) FeedShard {
+  uint32_t next;         // next unclaimed request
+  uint32_t pad0[63];
+  uint32_t done;         // requests published
+  uint32_t pad1[63];
+};
+
+// Donation demand (queued items minus idle wavefronts), kept as the sum of
+// kBalanceShards counters; wavefront w updates shard w % kBalanceShards.
+constexpr int kBalanceShards = 8;
+struct alignas(256) Balance {
+  int64_t v;
+  int64_t pad[31];
+};
+
+// Per-launch control block (reset by the host before each launch).  Only
+// the feed shards that own requests are copied; the kernel never reads the
+// others' counters.
+struct alignas(256) Control {
   uint32_t numJobs;
-  uint32_t pad0[30];
-  uint32_t jobsDone;
-  uint32_t pad1[31];
-  int32_t idle;          // wavefronts currently without work
-  uint32_t pad2[31];
-  uint64_t donations;
-  uint64_t pad3[15];
-  int64_t queued;        // items published and not yet claimed (all shards)
-  uint64_t pad5[15];
+  uint32_t perShard;
+  uint32_t pad0[62];
+  uint32_t drained;      // batched: 1 once some wavefront saw every feed shard exhausted
+  uint32_t pad1[63];
+  uint32_t shardsDone;   // batched: feed shards whose requests are all published
+  uint32_t pad2[63];
+  Balance balance[kBalanceShards];
+  uint32_t jobsDone;     // served: requests published
+  uint32_t pad4[63];
   // served loop
   uint64_t consumed;     // submit-ring entries taken by the device
   uint64_t completions;  // completion-ring entries written
   uint64_t lastPoll;     // device clock of the last host-ring poll
   uint64_t submittedMirror;
   uint64_t stopMirror;   // device copy of ServeHost::stop
-  uint64_t pad4[11];
+  uint64_t pad5[27];
+  FeedShard feed[kFeedShards];
 };
+
+// Fills the batched feed fields for `numJobs` requests (host side) and
+// returns how many leading bytes of the block a launch must copy.
+size_t init_feed(Control *c, uint32_t numJobs);
 
 // Host-visible rings of the served loop (fine-grained pinned host memory).
 struct alignas(128) ServeHost {

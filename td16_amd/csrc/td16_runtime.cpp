@@ -145,6 +145,26 @@ int alloc_buffers(Buffers &b, int device, uint32_t jobsCap, int64_t queueBytes, 
   return TD16_OK;
 }
 
+}  // namespace
+
+size_t init_feed(Control *c, uint32_t numJobs) {
+  c->numJobs = numJobs;
+  const uint32_t per = (numJobs + kFeedShards - 1) / kFeedShards;
+  c->perShard = per;
+  uint32_t used = 0;
+  for (int s = 0; s < kFeedShards; ++s) {
+    const uint64_t lo = std::min<uint64_t>((uint64_t)s * per, numJobs);
+    c->feed[s].next = (uint32_t)lo;
+    c->feed[s].done = 0;
+    if (lo < std::min<uint64_t>((uint64_t)(s + 1) * per, numJobs)) ++used;
+  }
+  c->shardsDone = (uint32_t)kFeedShards - used;  // empty shards count as done
+  // nonempty shards are the leading ones
+  return offsetof(Control, feed) + (size_t)used * sizeof(FeedShard);
+}
+
+namespace {
+
 Params make_params(const Buffers &b, const DeviceInfo &dev) {
   Params P;
   std::memset(&P, 0, sizeof(P));
@@ -168,7 +188,7 @@ void decode(const DevResult &r, td16_result *out) {
   out->status = TD16_OK;
   out->found = r.found;
   out->depth = r.found ? r.depth : 0;
-  out->split = r.split;
+  out->split = r.split != 0 ? 1 : 0;
   out->expanded = r.expanded;
   out->solutions = r.solutions;
   out->device_us = (r.t1 >= r.t0) ? (double)(r.t1 - r.t0) * 0.01 : 0.0;  // 100 MHz ticks
@@ -229,9 +249,9 @@ int run_plain(Slot &s, const td16_request *req, td16_result *res, char *msg, int
   TD16_HIP(hipSetDevice(s.device));
   s.buf.hJobs[0] = job;
   std::memset(s.buf.hCtrl, 0, sizeof(Control));
-  s.buf.hCtrl->numJobs = 1;
+  const size_t ctrlBytes = init_feed(s.buf.hCtrl, 1);
   TD16_HIP(hipMemcpyAsync(s.buf.dJobs, s.buf.hJobs, sizeof(Job), hipMemcpyHostToDevice, s.stream));
-  TD16_HIP(hipMemcpyAsync(s.buf.dCtrl, s.buf.hCtrl, sizeof(Control), hipMemcpyHostToDevice,
+  TD16_HIP(hipMemcpyAsync(s.buf.dCtrl, s.buf.hCtrl, ctrlBytes, hipMemcpyHostToDevice,
                           s.stream));
   Params P = make_params(s.buf, s.dev);
   TD16_HIP((hipError_t)launch_search(P, s.blocks, s.stream));
@@ -579,6 +599,25 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
     where.push_back(q);
   }
   const uint32_t nj = (uint32_t)jobs.size();
+  // Feed order: longest expected search first (move bound, then symbols).
+  // Heavy requests then start with the launch and are split once the feed
+  // drains, instead of starting late and becoming the launch's tail.
+  {
+    std::vector<uint32_t> order(nj);
+    for (uint32_t q = 0; q < nj; ++q) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      const Job &x = jobs[a], &y = jobs[b];
+      return x.maxDepth != y.maxDepth ? x.maxDepth > y.maxDepth : x.N > y.N;
+    });
+    std::vector<Job> sortedJobs(nj);
+    std::vector<int> sortedWhere(nj);
+    for (uint32_t q = 0; q < nj; ++q) {
+      sortedJobs[q] = jobs[order[q]];
+      sortedWhere[q] = where[order[q]];
+    }
+    jobs.swap(sortedJobs);
+    where.swap(sortedWhere);
+  }
 
   std::lock_guard<std::mutex> lk(gBatchMu);
   DeviceInfo dev;
@@ -623,7 +662,7 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
   std::memcpy(b.hJobs, jobs.data(), sizeof(Job) * nj);
   TD16_HIP(hipMemcpyAsync(b.dJobs, b.hJobs, sizeof(Job) * nj, hipMemcpyHostToDevice, gBatch.stream));
   std::memset(b.hCtrl, 0, sizeof(Control));
-  b.hCtrl->numJobs = nj;
+  const size_t ctrlBytes = init_feed(b.hCtrl, nj);
   Params P = make_params(b, dev);
   if (o.split_depth > 0) P.splitSlack = o.split_depth;
   if (o.donate_max > 0) P.donateMax = std::min(o.donate_max, 64);
@@ -632,10 +671,9 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
     reset_batch_graph();
     hipGraph_t g;
     TD16_HIP(hipStreamBeginCapture(gBatch.stream, hipStreamCaptureModeThreadLocal));
-    hipMemcpyAsync(b.dCtrl, b.hCtrl, sizeof(Control), hipMemcpyHostToDevice, gBatch.stream);
+    // the graph holds the kernel alone; the copies and the events that time
+    // the kernel stay outside it
     launch_search(P, blocks, gBatch.stream);
-    hipMemcpyAsync(b.hResults, b.dResults, sizeof(DevResult) * nj, hipMemcpyDeviceToHost,
-                   gBatch.stream);
     TD16_HIP(hipStreamEndCapture(gBatch.stream, &g));
     hipError_t e = hipGraphInstantiate(&gBatch.graph, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -645,22 +683,18 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
   }
 
   double kernelMs = 0.0, wallMs = 0.0;
-  Control endCtrl;
   for (int rep = 0; rep < o.repeats; ++rep) {
     const auto w0 = std::chrono::steady_clock::now();
-    if (o.use_graph) {
-      TD16_HIP(hipEventRecord(gBatch.ev0, gBatch.stream));
+    TD16_HIP(hipMemcpyAsync(b.dCtrl, b.hCtrl, ctrlBytes, hipMemcpyHostToDevice,
+                            gBatch.stream));
+    TD16_HIP(hipEventRecord(gBatch.ev0, gBatch.stream));
+    if (o.use_graph)
       TD16_HIP(hipGraphLaunch(gBatch.graph, gBatch.stream));
-      TD16_HIP(hipEventRecord(gBatch.ev1, gBatch.stream));
-    } else {
-      TD16_HIP(hipMemcpyAsync(b.dCtrl, b.hCtrl, sizeof(Control), hipMemcpyHostToDevice,
-                              gBatch.stream));
-      TD16_HIP(hipEventRecord(gBatch.ev0, gBatch.stream));
+    else
       TD16_HIP((hipError_t)launch_search(P, blocks, gBatch.stream));
-      TD16_HIP(hipEventRecord(gBatch.ev1, gBatch.stream));
-      TD16_HIP(hipMemcpyAsync(b.hResults, b.dResults, sizeof(DevResult) * nj,
-                              hipMemcpyDeviceToHost, gBatch.stream));
-    }
+    TD16_HIP(hipEventRecord(gBatch.ev1, gBatch.stream));
+    TD16_HIP(hipMemcpyAsync(b.hResults, b.dResults, sizeof(DevResult) * nj,
+                            hipMemcpyDeviceToHost, gBatch.stream));
     TD16_HIP(hipStreamSynchronize(gBatch.stream));
     const auto w1 = std::chrono::steady_clock::now();
     float ms = 0.f;
@@ -668,16 +702,15 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
     kernelMs += ms;
     wallMs += std::chrono::duration<double, std::milli>(w1 - w0).count();
   }
-  TD16_HIP(hipMemcpy(&endCtrl, b.dCtrl, sizeof(Control), hipMemcpyDeviceToHost));
   localStats.kernel_ms = kernelMs / o.repeats;
   localStats.wall_ms = wallMs / o.repeats;
-  localStats.donations = endCtrl.donations;
   for (uint32_t q = 0; q < nj; ++q) {
     const DevResult &r = b.hResults[q];
     td16_result &out = results[where[q]];
     decode(r, &out);
     localStats.expanded += r.expanded;
     localStats.solutions += r.solutions;
+    localStats.donations += (uint64_t)r.split;
   }
   if (stats) *stats = localStats;
   return TD16_OK;

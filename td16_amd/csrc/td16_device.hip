@@ -150,12 +150,21 @@ __device__ __forceinline__ int choose4(int m) {
 struct Frame {
   uint64_t pi;
   uint64_t mask[kMaxChunks];  // expandable children, bit t%64 of chunk t/64
-  uint8_t m;                  // enumerated elements (breakpoints or all N)
-  uint8_t full;               // 1: element q is position q
-  uint8_t cursor;             // first chunk that may still hold bits
-  uint8_t nchunks;
+  uint32_t hdr;               // frame header, see hdr_* below
   uint8_t pos[20];            // element -> position
 };
+
+// Frame header word: lexicographic table offset of the enumerated set
+// (C(m,4) <= 2380), chunk count, "element q is position q" flag and the first
+// chunk that may still hold bits.  One LDS word, read once per visit.
+__device__ __forceinline__ uint32_t hdr_make(int off, int nchunks, bool full, int cursor) {
+  return (uint32_t)off | ((uint32_t)nchunks << 12) | (full ? 1u << 16 : 0u) |
+         ((uint32_t)cursor << 24);
+}
+__device__ __forceinline__ int hdr_off(uint32_t h) { return (int)(h & 0xFFFu); }
+__device__ __forceinline__ int hdr_nchunks(uint32_t h) { return (int)((h >> 12) & 15u); }
+__device__ __forceinline__ bool hdr_full(uint32_t h) { return (h >> 16) & 1u; }
+__device__ __forceinline__ int hdr_cursor(uint32_t h) { return (int)(h >> 24); }
 
 struct WaveLds {
   Frame frames[kMaxDepth];
@@ -189,6 +198,12 @@ struct Ctx {
 };
 
 enum { kExpandOk = 0, kExpandFound = 1 };
+
+// Nodes after which an item counts as heavy: it gets raised wave priority
+// and, while unstarted requests remain, keeps up to kEarlyStock of its
+// subtrees queued.
+constexpr int kHeavyNodes = 256;
+constexpr int kEarlyStock = 512;
 
 // Packs the witness into the job state when this wave is the first finder.
 __device__ void record_witness(const Params &P, Ctx &c, WaveLds &w, int depth, int lastMove) {
@@ -261,8 +276,8 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
 
   // ---- enumeration set --------------------------------------------------
   const int cd = d + 1;
-  const uint32_t satMask = w.satTab[cd];
-  const uint32_t expMask = w.expTab[cd];
+  const uint32_t satMask = (uint32_t)uni((int)w.satTab[cd]);
+  const uint32_t expMask = (uint32_t)uni((int)w.expTab[cd]);
   const bool allMoves = (c.flags & kFlagAllMoves) != 0;
   bool full = false;
   if (allMoves) {
@@ -283,6 +298,7 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
     einfo = fperm(isBp ? r : 63, info);
     if (isBp) F.pos[r] = (uint8_t)p;
   }
+  m = uni(m);
   const int ncand = choose3(m);
   const int nchunks = (ncand + 63) >> 6;
   const int off = choose4(m);
@@ -296,37 +312,32 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
     const int A = bperm(e & 31, einfo);
     const int B = bperm((e >> 5) & 31, einfo);
     const int C = bperm((e >> 10) & 31, einfo);
-    const int Da = A & 31, La = (A >> 8) & 31, Ca = A >> 16;
-    const int Db = B & 31, Lb = (B >> 8) & 31, Cb = B >> 16;
-    const int Dc = C & 31, Lc = (C >> 8) & 31, Cc = C >> 16;
-    const bool sab = Ca == Cb, sbc = Cb == Cc, sac = Ca == Cc;
-    int delta;
-    bool acc;
-    if (sab && sbc) {
-      // one cycle: order a->b->c splits it in three, a->c->b keeps it
-      int dab = Da - Db; dab += dab < 0 ? La : 0;
-      int dbc = Db - Dc; dbc += dbc < 0 ? La : 0;
-      int dca = Dc - Da; dca += dca < 0 ? La : 0;
-      const int odd = (dab & 1) + (dbc & 1) + (dca & 1);
-      delta = (dab + dbc + dca == La) ? odd - (La & 1) : 0;
-      acc = true;
-    } else if (sab || sbc || sac) {
-      // x -> y on a shared cycle of length L1, the third symbol alone on L2;
-      // the x..y segment closes on itself, the rest joins the lone cycle
-      const int L1 = sab ? La : (sbc ? Lb : La);
-      const int L2 = sab ? Lc : (sbc ? La : Lb);
-      const int Dx = sab ? Da : (sbc ? Db : Dc);
-      const int Dy = sab ? Db : (sbc ? Dc : Da);
-      int dd = Dx - Dy; dd += dd < 0 ? L1 : 0;
-      const int l1 = L1 & 1, l2 = L2 & 1;
-      delta = (l1 == l2) ? 2 * (dd & 1) - 2 * l1 : 0;
-      acc = true;
-    } else {
-      // three cycles merge; the reference filter skips these
-      delta = ((La ^ Lb ^ Lc) & 1) - (La & 1) - (Lb & 1) - (Lc & 1);
-      acc = allMoves;
-    }
-    if (!allMoves) acc = acc && delta >= 0;
+    // Branch-free odd-cycle delta of the triple.  Arcs between the chosen
+    // positions along their cycles (meaningful for pairs on one cycle):
+    const int Da = A & 31, Db = B & 31, Dc = C & 31;
+    const int La = (A >> 8) & 31, Lb = (B >> 8) & 31, Lc = (C >> 8) & 31;
+    const int la = La & 1, lb = Lb & 1, lc = Lc & 1;
+    const bool sab = (A >> 16) == (B >> 16), sbc = (B >> 16) == (C >> 16),
+               sac = (A >> 16) == (C >> 16);
+    int dab = Da - Db; dab += dab < 0 ? La : 0;
+    int dbc = Db - Dc; dbc += dbc < 0 ? Lb : 0;
+    int dca = Dc - Da; dca += dca < 0 ? Lc : 0;
+    // one cycle: order a->b->c splits it into the arcs dab, dbc, dca,
+    // a->c->b keeps one cycle
+    const int d1 = (dab + dbc + dca == La) ? (dab & 1) + (dbc & 1) + (dca & 1) - la : 0;
+    // x -> y on a shared cycle (parity l1), the third position alone on a
+    // cycle of parity l2: the x..y arc closes on itself, the rest joins the
+    // lone cycle
+    const int pd = (sab ? dab : (sbc ? dbc : dca)) & 1;
+    const int l1 = sbc ? lb : la;
+    const int l2 = la ^ lb ^ lc;
+    const int d2 = pd + (l1 ^ l2 ^ pd) - l1 - l2;
+    // three cycles merge into one
+    const int d3 = (la ^ lb ^ lc) - la - lb - lc;
+    const bool any2 = sab || sbc || sac;
+    const int delta = (sab && sbc) ? d1 : (any2 ? d2 : d3);
+    // the reference filter skips three-cycle merges and negative deltas
+    const bool acc = allMoves || (any2 && delta >= 0);
     const int cec = (ec + delta) & 31;
     const bool isSat = valid && acc && ((satMask >> cec) & 1u);
     const bool isExp = valid && acc && !((satMask >> cec) & 1u) && ((expMask >> cec) & 1u);
@@ -355,10 +366,7 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   }
   if (lane == 0) {
     F.pi = pi;
-    F.m = (uint8_t)m;
-    F.full = full ? 1 : 0;
-    F.cursor = 0;
-    F.nchunks = (uint8_t)nchunks;
+    F.hdr = hdr_make(off, nchunks, full, 0);
   }
   wave_sync();
   return kExpandOk;
@@ -366,12 +374,11 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
 
 // Decodes bit `bit` of chunk `ch` of frame F into a packed move; every lane
 // decodes its own bit (no cross-lane step).
-__device__ __forceinline__ int frame_move_lane(const Frame &F, const uint16_t *lex, int ch,
-                                               int bit) {
-  const int m = F.m;
-  const int e = lex[choose4(m) + (ch << 6) + bit];
+__device__ __forceinline__ int frame_move_lane(const Frame &F, uint32_t hdr, const uint16_t *lex,
+                                               int ch, int bit) {
+  const int e = lex[hdr_off(hdr) + (ch << 6) + bit];
   int i = e & 31, j = (e >> 5) & 31, k = (e >> 10) & 31;
-  if (!F.full) {
+  if (!hdr_full(hdr)) {
     i = F.pos[i];
     j = F.pos[j];
     k = F.pos[k];
@@ -379,9 +386,21 @@ __device__ __forceinline__ int frame_move_lane(const Frame &F, const uint16_t *l
   return i | (j << 5) | (k << 10);
 }
 
-// Wave-uniform version: `ch` and `bit` are the same on every lane.
-__device__ __forceinline__ int frame_move(const Frame &F, const uint16_t *lex, int ch, int bit) {
-  return uni(frame_move_lane(F, lex, ch, bit));
+// Wave-uniform version (`hdr`, `ch` and `bit` are the same on every lane):
+// the lex entry and the position map are read at once, each lane holding
+// one position, and the three positions come back through readlane.
+__device__ __forceinline__ int frame_move(const Frame &F, uint32_t hdr, const uint16_t *lex,
+                                          int ch, int bit) {
+  const int lane = lane_id();
+  const int e = uni(lex[hdr_off(hdr) + (ch << 6) + bit]);
+  int i = e & 31, j = (e >> 5) & 31, k = (e >> 10) & 31;
+  if (!hdr_full(hdr)) {
+    const int pv = F.pos[lane < 20 ? lane : 0];
+    i = __builtin_amdgcn_readlane(pv, i);
+    j = __builtin_amdgcn_readlane(pv, j);
+    k = __builtin_amdgcn_readlane(pv, k);
+  }
+  return i | (j << 5) | (k << 10);
 }
 
 // ---------------------------------------------------------------------------
@@ -399,10 +418,10 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   const int maxFd = c.limit - P.splitSlack;
   int fd = -1, ch = -1;
   for (int q = c.depth0; q <= d && q <= maxFd; ++q) {
-    Frame &F = w.frames[q];
-    const int nch = F.nchunks;
-    for (int x = F.cursor; x < nch; ++x) {
-      if (F.mask[x] != 0) {
+    const uint32_t h = (uint32_t)uni((int)w.frames[q].hdr);
+    const int nch = hdr_nchunks(h);
+    for (int x = hdr_cursor(h); x < nch; ++x) {
+      if (uni64(w.frames[q].mask[x]) != 0) {
         fd = q;
         ch = x;
         break;
@@ -410,10 +429,9 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
     }
     if (fd >= 0) break;
   }
-  fd = uni(fd);
-  ch = uni(ch);
   if (fd < 0) return 0;
   Frame &F = w.frames[fd];
+  const uint32_t fhdr = (uint32_t)uni((int)F.hdr);
   uint64_t bits = uni64(F.mask[ch]);
   // keep at most `want` of them (the lowest bits go)
   if (want < 1) want = 1;
@@ -456,7 +474,7 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   }
   const uint64_t slot = (uint64_t)shard * cap + pos % cap;
   if (ok) {
-    const int mv = frame_move_lane(F, lex, ch, lane);
+    const int mv = frame_move_lane(F, fhdr, lex, ch, lane);
     Item it;
     it.pi = apply_move(F.pi, mv & 31, (mv >> 5) & 31, (mv >> 10) & 31);
     it.job = c.job;
@@ -605,10 +623,12 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
   if (!stop && expand(P, c, w, lex, depth0, pi) == kExpandFound) stop = true;
   int d = depth0;
   while (!stop) {
+    d = uni(d);
     Frame &F = w.frames[d];
     // next expandable child of the top frame
-    int ch = F.cursor;
-    const int nch = F.nchunks;
+    const uint32_t hdr = (uint32_t)uni((int)F.hdr);
+    int ch = hdr_cursor(hdr);
+    const int nch = hdr_nchunks(hdr);
     uint64_t bits = 0;
     while (ch < nch) {
       bits = uni64(F.mask[ch]);
@@ -621,11 +641,11 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
       continue;
     }
     const int bit = __builtin_ctzll(bits);
-    const int mv = frame_move(F, lex, ch, bit);
+    const int mv = frame_move(F, hdr, lex, ch, bit);
     const uint64_t child = apply_move(uni64(F.pi), mv & 31, (mv >> 5) & 31, (mv >> 10) & 31);
     if (lane == 0) {
       F.mask[ch] = bits & (bits - 1);
-      F.cursor = (uint8_t)ch;
+      F.hdr = (hdr & 0x00FFFFFFu) | ((uint32_t)ch << 24);
       w.path[d] = (uint16_t)mv;
     }
     wave_sync();
@@ -635,18 +655,25 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
     // parallelism keeps every wavefront busy for free.
     if (++c.sinceCheck >= 32) {
       c.sinceCheck = 0;
+      // an item that has proved heavy gets the SIMD's issue priority over
+      // the light requests sharing it, so it does not become the launch's
+      // tail; light work fills the remaining issue slots
+      if (c.expanded >= (uint64_t)kHeavyNodes) __builtin_amdgcn_s_setprio(2);
       const int want = lane0([&] {
-        // balance = queued items - idle wavefronts: donate only what idle
-        // wavefronts are not already being offered
+        // balance = queued items - idle wavefronts
         int64_t bal = 0;
 #pragma unroll
         for (int q = 0; q < kBalanceShards; ++q) bal += ld_rlx(&P.ctrl->balance[q].v);
-        if (bal >= 0) return 0;
         const bool drained = P.served
                                  ? ld_rlx(&P.ctrl->consumed) >= ld_rlx(&P.ctrl->submittedMirror)
                                  : ld_rlx(&P.ctrl->drained) != 0u;
-        if (!drained) return 0;
-        return -bal < (int64_t)P.donateMax ? (int)-bal : P.donateMax;
+        // requests still waiting: a heavy item keeps a small stock of its
+        // subtrees queued, which wavefronts take before new requests, so
+        // heavy requests spread early instead of crawling on one wavefront
+        // that shares its SIMD with light ones
+        int64_t room = drained ? -bal : (c.expanded >= (uint64_t)kHeavyNodes ? kEarlyStock - bal : 0);
+        if (room <= 0) return 0;
+        return room < (int64_t)P.donateMax ? (int)room : P.donateMax;
       });
       if (want > 0) donate(P, c, w, lex, d, want);
       if (c.shared && !countAll) {
@@ -656,6 +683,7 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
     ++d;
     if (expand(P, c, w, lex, d, child) == kExpandFound) break;
   }
+  __builtin_amdgcn_s_setprio(0);
   // flush counters, then release this item (the release publishes the
   // counters and any witness to the wavefront that finalizes the request)
   if (!c.shared) {
@@ -824,7 +852,10 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(kBlockThreads)
+// Register budget for five waves per SIMD: the code the cap pushes to
+// scratch is per-request set-up, not the per-node path, and the measured
+// mix runs fastest with it at three resident waves per SIMD.
+__global__ void __launch_bounds__(kBlockThreads, 5)
 td16_search_kernel(Params P) {
   __shared__ BlockLds lds;
   for (int q = threadIdx.x; q < kLexEntries; q += blockDim.x) lds.lex[q] = P.lex[q];

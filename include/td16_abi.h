@@ -84,9 +84,12 @@ typedef struct {
  *   moves: positions (i, j, k) of the witness, moves[0..depth-1]
  *   expanded: nodes whose C(n,3) children were evaluated (root included)
  *   solutions: witnesses met (COUNT_ALL: all of them)
- *   split: 1 when the search was shared by several wavefronts (its FIRST
- *     witness and counts may then differ from a sequential DFS's; found /
- *     not-found never does)
+ *   split: 1 when the search left the sequential lexicographic DFS: it was
+ *     shared by several wavefronts, or (FIRST mode) it proved heavy and its
+ *     later frames try moves that gain two odd cycles first.  Its FIRST
+ *     witness and node count may then differ from a sequential DFS's; the
+ *     witness is still valid and found / not-found never differs (a complete
+ *     search expands the same node set in any order)
  *   device_us: device time from the request's start on a wavefront to its
  *     result's publication (the chip's 100 MHz real-time counter)
  *   latency_us: served loop only, submit (or scheduled arrival) to the

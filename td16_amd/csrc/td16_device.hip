@@ -147,24 +147,39 @@ __device__ __forceinline__ int choose4(int m) {
 // Per-wave LDS state
 // --------------------------------------------------------------------------
 
+// A frame lists its expandable children as bitmasks over the lexicographic
+// candidate index t (bit t%64 of chunk t/64).  In a reordered frame the
+// chunks come twice: first the children that gain two odd cycles, then the
+// others, so a visit in chunk order tries the greedy moves first.
 struct Frame {
   uint64_t pi;
-  uint64_t mask[kMaxChunks];  // expandable children, bit t%64 of chunk t/64
+  uint64_t mask[2 * kMaxChunks];
   uint32_t hdr;               // frame header, see hdr_* below
   uint8_t pos[20];            // element -> position
 };
 
 // Frame header word: lexicographic table offset of the enumerated set
-// (C(m,4) <= 2380), chunk count, "element q is position q" flag and the first
-// chunk that may still hold bits.  One LDS word, read once per visit.
-__device__ __forceinline__ uint32_t hdr_make(int off, int nchunks, bool full, int cursor) {
+// (C(m,4) <= 2380), candidate chunk count, "element q is position q" flag,
+// "reordered" flag and the first mask chunk that may still hold bits.  One
+// LDS word, read once per visit.
+__device__ __forceinline__ uint32_t hdr_make(int off, int nchunks, bool full, bool reordered,
+                                             int cursor) {
   return (uint32_t)off | ((uint32_t)nchunks << 12) | (full ? 1u << 16 : 0u) |
-         ((uint32_t)cursor << 24);
+         (reordered ? 1u << 17 : 0u) | ((uint32_t)cursor << 24);
 }
 __device__ __forceinline__ int hdr_off(uint32_t h) { return (int)(h & 0xFFFu); }
 __device__ __forceinline__ int hdr_nchunks(uint32_t h) { return (int)((h >> 12) & 15u); }
 __device__ __forceinline__ bool hdr_full(uint32_t h) { return (h >> 16) & 1u; }
+__device__ __forceinline__ bool hdr_reordered(uint32_t h) { return (h >> 17) & 1u; }
 __device__ __forceinline__ int hdr_cursor(uint32_t h) { return (int)(h >> 24); }
+// mask chunks to scan, and the candidate chunk a mask chunk stands for
+__device__ __forceinline__ int hdr_mask_chunks(uint32_t h) {
+  return hdr_reordered(h) ? 2 * hdr_nchunks(h) : hdr_nchunks(h);
+}
+__device__ __forceinline__ int hdr_cand_chunk(uint32_t h, int ch) {
+  const int n = hdr_nchunks(h);
+  return ch >= n ? ch - n : ch;
+}
 
 struct WaveLds {
   Frame frames[kMaxDepth];
@@ -195,6 +210,9 @@ struct Ctx {
   int home;        // this wave's queue shard and feed shard
   uint32_t rng;    // LCG state for random shard probes
   bool drained;    // batched: this wave saw every feed shard exhausted
+  bool reorder;    // the request proved heavy: new frames visit greedy children first
+  bool gave;       // this item donated subtrees
+  uint64_t discard;  // nodes of this item expanded before a greedy restart
 };
 
 enum { kExpandOk = 0, kExpandFound = 1 };
@@ -362,11 +380,20 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
         return kExpandFound;
       }
     }
-    if (lane == 0) F.mask[ch] = expb;
+    // reordered frame: the greedy children (two more odd cycles) first
+    const uint64_t greedy = c.reorder ? ballot(isExp && delta > 0) : 0ULL;
+    if (lane == 0) {
+      if (c.reorder) {
+        F.mask[ch] = greedy;
+        F.mask[nchunks + ch] = expb & ~greedy;
+      } else {
+        F.mask[ch] = expb;
+      }
+    }
   }
   if (lane == 0) {
     F.pi = pi;
-    F.hdr = hdr_make(off, nchunks, full, 0);
+    F.hdr = hdr_make(off, nchunks, full, c.reorder, 0);
   }
   wave_sync();
   return kExpandOk;
@@ -376,7 +403,7 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
 // decodes its own bit (no cross-lane step).
 __device__ __forceinline__ int frame_move_lane(const Frame &F, uint32_t hdr, const uint16_t *lex,
                                                int ch, int bit) {
-  const int e = lex[hdr_off(hdr) + (ch << 6) + bit];
+  const int e = lex[hdr_off(hdr) + (hdr_cand_chunk(hdr, ch) << 6) + bit];
   int i = e & 31, j = (e >> 5) & 31, k = (e >> 10) & 31;
   if (!hdr_full(hdr)) {
     i = F.pos[i];
@@ -392,7 +419,7 @@ __device__ __forceinline__ int frame_move_lane(const Frame &F, uint32_t hdr, con
 __device__ __forceinline__ int frame_move(const Frame &F, uint32_t hdr, const uint16_t *lex,
                                           int ch, int bit) {
   const int lane = lane_id();
-  const int e = uni(lex[hdr_off(hdr) + (ch << 6) + bit]);
+  const int e = uni(lex[hdr_off(hdr) + (hdr_cand_chunk(hdr, ch) << 6) + bit]);
   int i = e & 31, j = (e >> 5) & 31, k = (e >> 10) & 31;
   if (!hdr_full(hdr)) {
     const int pv = F.pos[lane < 20 ? lane : 0];
@@ -419,7 +446,7 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   int fd = -1, ch = -1;
   for (int q = c.depth0; q <= d && q <= maxFd; ++q) {
     const uint32_t h = (uint32_t)uni((int)w.frames[q].hdr);
-    const int nch = hdr_nchunks(h);
+    const int nch = hdr_mask_chunks(h);
     for (int x = hdr_cursor(h); x < nch; ++x) {
       if (uni64(w.frames[q].mask[x]) != 0) {
         fd = q;
@@ -465,6 +492,7 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   const uint64_t okBits = ballot(ok);
   const int cnt = __builtin_popcountll(okBits);
   if (cnt == 0) return 0;
+  c.gave = true;
   // account before publishing so the request cannot complete underneath;
   // the release fence below orders these before any consumer's claim
   if (lane == 0) {
@@ -493,30 +521,30 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   return cnt;
 }
 
-// Claims one item: the home shard first, then one pseudo-random shard.
-// Returns 1 and fills `it` (and the wave's path prefix) when there was one.
+// Claims one item from the home shard or one pseudo-random shard: lanes 0
+// and 1 probe one shard each at the same time (head, then the head slot's
+// sequence number), and lane 0 claims the first published slot with a CAS.
+// Returns 1 and fills `it` (and the wave's path prefix) when it got one.
 __device__ int pop_item(const Params &P, Ctx &c, Item &it, WaveLds &w) {
   const int lane = lane_id();
   const uint64_t cap = P.qcap;
   c.rng = c.rng * 1664525u + 1013904223u;
   const int other = (int)((c.rng >> 8) % kQueueShards);
-  int got = 0, shard = 0;
-  uint64_t h = 0;
-  if (lane == 0) {
+  const int myShard = lane == 0 ? c.home : other;
+  bool ready = false;
+  uint64_t hl = 0;
+  if (lane < 2) {
     // relaxed polling: the acquire fence is paid only once a slot is claimed
-    for (int probe = 0; probe < 2 && !got; ++probe) {
-      shard = probe == 0 ? c.home : other;
-      QueueState *q = &P.queue[shard];
-      const uint64_t *sq = P.seq + (uint64_t)shard * cap;
-      h = ld_rlx(&q->head);
-      const uint64_t s = ld_rlx(&sq[h % cap]);
-      if (s == h + 1 && cas_rlx(&q->head, h, h + 1)) got = 1;
-    }
+    hl = ld_rlx(&P.queue[myShard].head);
+    ready = ld_rlx(&P.seq[(uint64_t)myShard * cap + hl % cap]) == hl + 1;
   }
-  got = uni(got);
-  if (!got) return 0;
-  shard = uni(shard);
-  h = uni64(h);
+  const uint64_t rdy = ballot(ready);
+  if (rdy == 0) return 0;
+  const int src = __builtin_ctzll(rdy);
+  const int shard = __builtin_amdgcn_readlane(myShard, src);
+  const uint64_t h = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(hl >> 32), src) << 32) |
+                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hl, src);
+  if (lane0([&] { return cas_rlx(&P.queue[shard].head, h, h + 1) ? 1 : 0; }) == 0) return 0;
   fence_acq();
   const uint64_t slot = (uint64_t)shard * cap + h % cap;
   it = P.items[slot];
@@ -606,6 +634,8 @@ __device__ void load_job(const Params &P, Ctx &c, WaveLds &w, uint32_t j, int li
   wave_sync();
 }
 
+__device__ bool release_item(const Params &P, Ctx &c);
+
 // Runs one item (a node to expand and its whole subtree) to completion.
 // Returns true when this wave released the request's last pending item.
 __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
@@ -613,6 +643,8 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
   const int lane = lane_id();
   c.depth0 = depth0;
   c.expanded = 0;
+  c.discard = 0;
+  c.gave = false;
   c.solutions = 0;
   c.sinceCheck = 0;
   const bool countAll = (c.flags & kFlagCountAll) != 0;
@@ -628,7 +660,7 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
     // next expandable child of the top frame
     const uint32_t hdr = (uint32_t)uni((int)F.hdr);
     int ch = hdr_cursor(hdr);
-    const int nch = hdr_nchunks(hdr);
+    const int nch = hdr_mask_chunks(hdr);
     uint64_t bits = 0;
     while (ch < nch) {
       bits = uni64(F.mask[ch]);
@@ -658,7 +690,26 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
       // an item that has proved heavy gets the SIMD's issue priority over
       // the light requests sharing it, so it does not become the launch's
       // tail; light work fills the remaining issue slots
-      if (c.expanded >= (uint64_t)kHeavyNodes) __builtin_amdgcn_s_setprio(2);
+      if (c.expanded >= (uint64_t)kHeavyNodes) {
+        __builtin_amdgcn_s_setprio(2);
+        // a FIRST search that proved heavy leaves the lexicographic order:
+        // this item restarts from its own root with frames that try the
+        // moves gaining two odd cycles first, and so does every item of the
+        // request opened from now on
+        if (!c.reorder && !countAll) {
+          c.reorder = true;
+          if (lane == 0) st_rlx(&st->reorder, 1u);
+          // restart only an item that has given nothing away, and count
+          // its nodes once: an exhaustive search then still expands (and
+          // reports) exactly the sequential search's node set
+          if (!c.gave) {
+            c.discard = c.expanded;
+            d = depth0;
+            if (expand(P, c, w, lex, d, uni64(w.frames[depth0].pi)) == kExpandFound) break;
+            continue;
+          }
+        }
+      }
       const int want = lane0([&] {
         // balance = queued items - idle wavefronts
         int64_t bal = 0;
@@ -684,13 +735,21 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
     if (expand(P, c, w, lex, d, child) == kExpandFound) break;
   }
   __builtin_amdgcn_s_setprio(0);
-  // flush counters, then release this item (the release publishes the
-  // counters and any witness to the wavefront that finalizes the request)
+  return release_item(P, c);
+}
+
+// Flushes the item's counters and releases it (the release publishes the
+// counters and any witness to the wavefront that finalizes the request).
+// Returns true when this wave released the request's last pending item.
+__device__ bool release_item(const Params &P, Ctx &c) {
+  const int lane = lane_id();
+  JobState *st = &P.states[c.job];
+  const uint64_t expanded = c.expanded - c.discard;
   if (!c.shared) {
     // private request (root item, never donated from): this wave is the
     // only one that ever held it, so it finalizes with plain stores
     if (lane == 0) {
-      st->expanded += c.expanded;
+      st->expanded += expanded;
       st->solutions += c.solutions;
       st->pending = 0;
     }
@@ -699,7 +758,7 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
   }
   int last = 0;
   if (lane == 0) {
-    if (c.expanded) add_rlx(&st->expanded, c.expanded);
+    if (expanded) add_rlx(&st->expanded, expanded);
     if (c.solutions) add_rlx(&st->solutions, c.solutions);
     last = __hip_atomic_fetch_add(&st->pending, -1, __ATOMIC_RELEASE,
                                   __HIP_MEMORY_SCOPE_AGENT) == 1 ? 1 : 0;
@@ -721,12 +780,15 @@ __device__ __forceinline__ uint32_t result_word(const Params &P, uint32_t j, int
   else if (q == 16 || q == 17) src = q;       // t0
   if (src >= 0) return st[src];
   if (q == 13) return P.jobs[j].tag;
+  if (q == 15) return st[14];                 // reorder flag
   if (q == 18 || q == 19) {
     const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
     return q == 18 ? (uint32_t)t1 : (uint32_t)(t1 >> 32);
   }
   return 0u;  // seq, pad
 }
+static_assert(offsetof(JobState, reorder) == 56 && offsetof(DevResult, reordered) == 60,
+              "result_word copies the reorder flag by word");
 static_assert(offsetof(JobState, found) == 4 && offsetof(JobState, limit) == 8 &&
                   offsetof(JobState, split) == 12 && offsetof(JobState, expanded) == 16 &&
                   offsetof(JobState, depth) == 32 && offsetof(JobState, moves) == 36 &&
@@ -781,6 +843,7 @@ __device__ int start_job(const Params &P, uint32_t j) {
     st->expanded = 0;
     st->solutions = 0;
     st->depth = 0;
+    st->reorder = 0;
     st->t0 = __builtin_amdgcn_s_memrealtime();
   }
   // no fence: until this wave donates (which releases), no other wavefront
@@ -871,12 +934,25 @@ td16_search_kernel(Params P) {
   c.home = (int)((blockIdx.x * kWavesPerBlock + wid) % kQueueShards);
   c.rng = 0x9E3779B9u * (blockIdx.x * kWavesPerBlock + wid + 1);
   c.drained = false;
+  c.reorder = false;
   bool idle = false;
   int backoff = 1;
   for (;;) {
     Item it;
     uint32_t j = 0;
-    bool haveItem = pop_item(P, c, it, w) != 0;
+    // While the batched feed lasts no wavefront idles, so the balance
+    // counters hold the number of queued items: one round trip over them
+    // spares the probes of an empty queue.
+    bool tryPop = true;
+    if (!P.served && !c.drained) {
+      tryPop = lane0([&] {
+        int64_t bal = 0;
+#pragma unroll
+        for (int q = 0; q < kBalanceShards; ++q) bal += ld_rlx(&P.ctrl->balance[q].v);
+        return bal > 0 ? 1 : 0;
+      }) != 0;
+    }
+    bool haveItem = tryPop && pop_item(P, c, it, w) != 0;
     bool haveJob = false;
     if (!haveItem) {
       if (!P.served) {
@@ -930,12 +1006,27 @@ td16_search_kernel(Params P) {
       pi = uni64(P.jobs[j].pi0);
       depth0 = 0;
       c.shared = false;
+      c.reorder = false;
     } else {
       j = (uint32_t)uni((int)it.job);
+      c.job = j;
+      c.shared = true;
+      // a request that already has its witness: release the item unopened
+      // (COUNT_ALL requests never set found)
+      const int flags_ = lane0([&] {
+        return (int)ld_rlx(&P.states[j].found) | ((int)ld_rlx(&P.states[j].reorder) << 1);
+      });
+      c.reorder = (flags_ & 2) != 0;
+      if ((flags_ & 1) != 0) {
+        c.expanded = 0;
+        c.discard = 0;
+        c.solutions = 0;
+        if (release_item(P, c)) publish_result(P, j);
+        continue;
+      }
       pi = uni64(it.pi);
       depth0 = uni((int)it.depth);
       limit = uni(ld_rlx(&P.states[j].limit));  // ordered by pop_item's acquire
-      c.shared = true;
     }
     load_job(P, c, w, j, limit);
     for (;;) {
@@ -961,6 +1052,7 @@ td16_search_kernel(Params P) {
       pi = uni64(P.jobs[j].pi0);
       depth0 = 0;
       c.shared = false;
+      c.reorder = lane0([&] { return (int)st->reorder; }) != 0;
     }
   }
 }

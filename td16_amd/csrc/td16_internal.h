@@ -56,7 +56,7 @@ struct alignas(64) JobState {
   uint64_t solutions;
   int32_t depth;         // witness length
   uint16_t moves[10];    // witness, packed i | j<<5 | k<<10
-  uint8_t pad[4];
+  uint32_t reorder;      // 1 once the FIRST search proved heavy and left the lexicographic order
   uint64_t t0;           // device real-time counter (100 MHz) when the request started
   uint8_t pad2[56];
 };
@@ -73,7 +73,7 @@ struct alignas(16) DevResult {
   uint16_t moves[10];
   uint32_t tag;
   uint32_t seq;          // served loop: completion sequence + 1 once written
-  uint8_t pad[4];
+  uint32_t reordered;    // the search left the lexicographic order
   uint64_t t0, t1;       // device real-time counter at start and at publication
 };
 static_assert(sizeof(DevResult) == 80, "DevResult layout");

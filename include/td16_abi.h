@@ -62,11 +62,18 @@ enum { TD16_FIRST = 0, TD16_COUNT_ALL = 1 };
  *   iddfs: 1 = exact distance by iterative deepening on limits
  *     max(ceil(b/3), odd-cycle bound), +1, ... up to max_moves (min_rate,
  *     initial_even and full_sorting are then ignored: the witness condition
- *     is "sorted"). */
+ *     is "sorted").
+ *   lex_order: visiting order of a FIRST search's expandable children.
+ *     0 (default) = greedy: the children that gain two odd cycles first, then
+ *     the others, each class in the reference's lexicographic triple order;
+ *     1 = lexicographic, the reference's order.  A node's children are always
+ *     evaluated in lexicographic order, so the witness move found at a node
+ *     is the lexicographically first one in both.  COUNT_ALL ignores it. */
 typedef struct {
     uint8_t beta[TD16_MAX_N];
     uint8_t omega[TD16_MAX_N];
-    uint8_t pad0[2];
+    uint8_t lex_order;
+    uint8_t pad0;
     int32_t n;
     int32_t initial_even;
     float min_rate;
@@ -84,12 +91,10 @@ typedef struct {
  *   moves: positions (i, j, k) of the witness, moves[0..depth-1]
  *   expanded: nodes whose C(n,3) children were evaluated (root included)
  *   solutions: witnesses met (COUNT_ALL: all of them)
- *   split: 1 when the search left the sequential lexicographic DFS: it was
- *     shared by several wavefronts, or (FIRST mode) it proved heavy and its
- *     later frames try moves that gain two odd cycles first.  Its FIRST
- *     witness and node count may then differ from a sequential DFS's; the
- *     witness is still valid and found / not-found never differs (a complete
- *     search expands the same node set in any order)
+ *   split: 1 when the search was shared by several wavefronts.  Its FIRST
+ *     witness and node count may then differ from a sequential DFS's in the
+ *     same order; the witness is still valid and found / not-found never
+ *     differs (a complete search expands the same node set in any order)
  *   device_us: device time from the request's start on a wavefront to its
  *     result's publication (the chip's 100 MHz real-time counter)
  *   latency_us: served loop only, submit (or scheduled arrival) to the

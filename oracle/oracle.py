@@ -58,6 +58,11 @@ def lib():
         L.td16o_search.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.POINTER(Result)]
+        L.td16o_search_order.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.POINTER(Result)]
+        L.td16o_iddfs_order.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.POINTER(Result)]
         L.td16o_search_tables.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, u32p, u32p,
                                           ctypes.c_int, ctypes.c_int, ctypes.POINTER(Result)]
         L.td16o_iddfs.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -87,7 +92,8 @@ def lib():
         L.td16o_perm_unrank.argtypes = [ctypes.c_uint64, ctypes.c_int, u8p]
         L.td16o_perm_unrank.restype = None
         L.td16o_search_batch.argtypes = [ctypes.c_int, u8p, u64p, u8p, ctypes.c_int,
-                                         ctypes.c_int, ctypes.c_int, ctypes.POINTER(Result)]
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(Result)]
         _lib = L
     return _lib
 
@@ -114,10 +120,13 @@ def rate_tables(n, initial_even, min_rate, max_depth, full_sorting):
 
 
 def search(beta, omega, initial_even, min_rate, max_depth, full_sorting=True,
-           moveset=MOVES_REF, mode=FIRST):
+           moveset=MOVES_REF, mode=FIRST, greedy=False):
+    """greedy=False: the reference's lexicographic DFS; greedy=True: children
+    gaining odd cycles first (the kernel's default FIRST order)."""
     r = Result()
-    rc = lib().td16o_search(_u8(beta), _u8(omega), len(beta), initial_even, min_rate,
-                            max_depth, int(full_sorting), moveset, mode, ctypes.byref(r))
+    rc = lib().td16o_search_order(_u8(beta), _u8(omega), len(beta), initial_even, min_rate,
+                                  max_depth, int(full_sorting), moveset, mode, int(greedy),
+                                  ctypes.byref(r))
     if rc != 0:
         raise ValueError("invalid search input")
     return r
@@ -133,9 +142,10 @@ def search_tables(beta, omega, max_depth, sat, reach, moveset=MOVES_REF, mode=FI
     return r
 
 
-def iddfs(beta, omega, moveset=MOVES_REF, max_limit=9):
+def iddfs(beta, omega, moveset=MOVES_REF, max_limit=9, greedy=False):
     r = Result()
-    d = lib().td16o_iddfs(_u8(beta), _u8(omega), len(beta), moveset, max_limit, ctypes.byref(r))
+    d = lib().td16o_iddfs_order(_u8(beta), _u8(omega), len(beta), moveset, max_limit,
+                                int(greedy), ctypes.byref(r))
     return d, r
 
 
@@ -265,7 +275,7 @@ def perm_rank(p):
     return lib().td16o_perm_rank(_u8(p), len(p))
 
 
-def search_batch(ns, perms, ks, moveset=MOVES_REF, mode=FIRST, threads=1):
+def search_batch(ns, perms, ks, moveset=MOVES_REF, mode=FIRST, threads=1, greedy=False):
     count = len(ns)
     out = (Result * count)()
     ns = np.ascontiguousarray(ns, dtype=np.uint8)
@@ -274,5 +284,5 @@ def search_batch(ns, perms, ks, moveset=MOVES_REF, mode=FIRST, threads=1):
     lib().td16o_search_batch(count, ns.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                              perms.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                              ks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-                             moveset, mode, threads, out)
+                             moveset, mode, int(greedy), threads, out)
     return out

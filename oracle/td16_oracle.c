@@ -157,7 +157,7 @@ static void apply_symbols(uint8_t *omega, const uint8_t *parent, int a, int b, i
 /* ------------------------------------------------------------------ */
 
 typedef struct {
-    int n, maxDepth, moveset, mode;
+    int n, maxDepth, moveset, mode, greedy;
     uint32_t sat[10], reach[10];
     td16o_result *out;
     int stop;
@@ -223,7 +223,11 @@ static void expand(search_ctx *cx, const uint8_t *beta, const uint8_t *omega, in
             }
         }
     }
+    /* greedy order: children that gain odd cycles first, then the rest,
+     * each class in lexicographic order (one pass in lexicographic order) */
+    for (int pass = 0; pass < (cx->greedy ? 2 : 1); ++pass)
     for (int q = 0; q < nk; ++q) {
+        if (cx->greedy && (pass == 0) != (kec[q] > ec)) continue;
         uint8_t cb[TD16O_MAX_N], co[TD16O_MAX_N];
         const int i = ki[q], j = kj[q], k = kk[q];
         memcpy(cb, beta, (size_t)n);
@@ -250,15 +254,25 @@ static int valid_pair(const uint8_t *beta, const uint8_t *omega, int n) {
     return sb == full && so == full;
 }
 
+static int search_impl(const uint8_t *beta, const uint8_t *omega, int n, int maxDepth,
+                       const uint32_t sat[10], const uint32_t reach[10], int moveset,
+                       int mode, int greedy, uint64_t cap, td16o_result *out);
+
 int td16o_search_tables(const uint8_t *beta, const uint8_t *omega, int n, int maxDepth,
                         const uint32_t sat[10], const uint32_t reach[10], int moveset,
                         int mode, td16o_result *out) {
-    return td16o_search_capped(beta, omega, n, maxDepth, sat, reach, moveset, mode, 0, out);
+    return search_impl(beta, omega, n, maxDepth, sat, reach, moveset, mode, 0, 0, out);
 }
 
 int td16o_search_capped(const uint8_t *beta, const uint8_t *omega, int n, int maxDepth,
                         const uint32_t sat[10], const uint32_t reach[10], int moveset,
                         int mode, uint64_t cap, td16o_result *out) {
+    return search_impl(beta, omega, n, maxDepth, sat, reach, moveset, mode, 0, cap, out);
+}
+
+static int search_impl(const uint8_t *beta, const uint8_t *omega, int n, int maxDepth,
+                       const uint32_t sat[10], const uint32_t reach[10], int moveset,
+                       int mode, int greedy, uint64_t cap, td16o_result *out) {
     memset(out, 0, sizeof(*out));
     if (!valid_pair(beta, omega, n)) return -1;
     if (maxDepth < 1 || maxDepth > TD16O_MAX_DEPTH) return -1;
@@ -269,6 +283,7 @@ int td16o_search_capped(const uint8_t *beta, const uint8_t *omega, int n, int ma
     cx.maxDepth = maxDepth;
     cx.moveset = moveset;
     cx.mode = mode;
+    cx.greedy = greedy;
     memcpy(cx.sat, sat, sizeof(cx.sat));
     memcpy(cx.reach, reach, sizeof(cx.reach));
     cx.out = out;
@@ -285,8 +300,21 @@ int td16o_search(const uint8_t *beta, const uint8_t *omega, int n, int initialEv
     return td16o_search_tables(beta, omega, n, maxDepth, sat, reach, moveset, mode, out);
 }
 
+int td16o_search_order(const uint8_t *beta, const uint8_t *omega, int n, int initialEven,
+                       float minRate, int maxDepth, int fullSorting, int moveset, int mode,
+                       int greedy, td16o_result *out) {
+    uint32_t sat[10], reach[10];
+    td16o_rate_tables(n, initialEven, minRate, maxDepth, fullSorting, sat, reach);
+    return search_impl(beta, omega, n, maxDepth, sat, reach, moveset, mode, greedy, 0, out);
+}
+
 int td16o_iddfs(const uint8_t *beta, const uint8_t *omega, int n, int moveset, int maxLimit,
                 td16o_result *out) {
+    return td16o_iddfs_order(beta, omega, n, moveset, maxLimit, 0, out);
+}
+
+int td16o_iddfs_order(const uint8_t *beta, const uint8_t *omega, int n, int moveset,
+                      int maxLimit, int greedy, td16o_result *out) {
     memset(out, 0, sizeof(*out));
     if (!valid_pair(beta, omega, n)) return -1;
     const int odd = td16o_count_odd_cycles(omega, n);
@@ -301,7 +329,7 @@ int td16o_iddfs(const uint8_t *beta, const uint8_t *omega, int n, int moveset, i
         uint32_t sat[10], reach[10];
         td16o_rate_tables(n, odd, 0.0f, limit, 1, sat, reach);
         td16o_result r;
-        td16o_search_tables(beta, omega, n, limit, sat, reach, moveset, TD16O_FIRST, &r);
+        search_impl(beta, omega, n, limit, sat, reach, moveset, TD16O_FIRST, greedy, 0, &r);
         total += r.expanded;
         if (r.found) {
             *out = r;
@@ -721,7 +749,7 @@ typedef struct {
     const uint8_t *ns;
     const uint64_t *perms;
     const uint8_t *ks;
-    int moveset, mode;
+    int moveset, mode, greedy;
     td16o_result *out;
     int next;
     pthread_mutex_t mu;
@@ -743,7 +771,7 @@ static void *batch_worker(void *arg) {
         /* caller convention E/UnorientedCycles.java:34 */
         const float minRate = 2.0f / ((float)k / (float)target);
         td16o_result *r = &b->out[idx];
-        td16o_search(beta, omega, N, odd, minRate, k, 1, b->moveset, b->mode, r);
+        td16o_search_order(beta, omega, N, odd, minRate, k, 1, b->moveset, b->mode, b->greedy, r);
         if (r->found) {
             int32_t sym[27];
             td16o_positions_to_symbols(beta, N, &r->moves[0][0], r->depth, sym);
@@ -754,7 +782,7 @@ static void *batch_worker(void *arg) {
 }
 
 int td16o_search_batch(int count, const uint8_t *ns, const uint64_t *perms, const uint8_t *ks,
-                       int moveset, int mode, int threads, td16o_result *out) {
+                       int moveset, int mode, int greedy, int threads, td16o_result *out) {
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;
     batch_ctx b;
@@ -764,6 +792,7 @@ int td16o_search_batch(int count, const uint8_t *ns, const uint64_t *perms, cons
     b.ks = ks;
     b.moveset = moveset;
     b.mode = mode;
+    b.greedy = greedy;
     b.out = out;
     b.next = 0;
     pthread_mutex_init(&b.mu, NULL);

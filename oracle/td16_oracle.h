@@ -74,6 +74,18 @@ int td16o_search(const uint8_t *beta, const uint8_t *omega, int n,
                  int initialEven, float minRate, int maxDepth,
                  int fullSorting, int moveset, int mode, td16o_result *out);
 
+/* Same search in greedy order (greedy = 1): after a node's children are
+ * evaluated, the expandable ones that gain odd cycles are searched first,
+ * then the others, each class in lexicographic order.  This is the kernel's
+ * default FIRST order; it is this build's extension, not the reference's.
+ * greedy = 0 is td16o_search.  Pinned: exhaustive searches expand the same
+ * node set in either order (equal counts), found / not-found agree, and a
+ * separate Python restatement gives the same witnesses on small inputs. */
+int td16o_search_order(const uint8_t *beta, const uint8_t *omega, int n,
+                       int initialEven, float minRate, int maxDepth,
+                       int fullSorting, int moveset, int mode, int greedy,
+                       td16o_result *out);
+
 /* Same search with explicit predicate tables (the IDDFS driver uses it). */
 int td16o_search_tables(const uint8_t *beta, const uint8_t *omega, int n,
                         int maxDepth, const uint32_t sat[10],
@@ -93,6 +105,9 @@ int td16o_search_capped(const uint8_t *beta, const uint8_t *omega, int n,
  * expanded counts over all iterations. */
 int td16o_iddfs(const uint8_t *beta, const uint8_t *omega, int n, int moveset,
                 int maxLimit, td16o_result *out);
+/* The same in either search order (greedy as in td16o_search_order). */
+int td16o_iddfs_order(const uint8_t *beta, const uint8_t *omega, int n, int moveset,
+                      int maxLimit, int greedy, td16o_result *out);
 
 /* Naive enumeration: every sequence of every triple with a full recount at
  * each node, no delta and no early pruning order; counts what td16o_search
@@ -162,8 +177,8 @@ void td16o_perm_unrank(uint64_t r, int n, uint8_t *p);
  * (one instance per thread at a time); used as the CPU baseline.
  * Instances are linear (ns, perms); k is the move bound per instance. */
 int td16o_search_batch(int count, const uint8_t *ns, const uint64_t *perms,
-                       const uint8_t *ks, int moveset, int mode, int threads,
-                       td16o_result *out);
+                       const uint8_t *ks, int moveset, int mode, int greedy,
+                       int threads, td16o_result *out);
 
 #ifdef __cplusplus
 }

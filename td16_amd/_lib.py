@@ -52,7 +52,8 @@ class Request(ctypes.Structure):
     _fields_ = [
         ("beta", ctypes.c_uint8 * MAX_N),
         ("omega", ctypes.c_uint8 * MAX_N),
-        ("pad0", ctypes.c_uint8 * 2),
+        ("lex_order", ctypes.c_uint8),
+        ("pad0", ctypes.c_uint8),
         ("n", ctypes.c_int32),
         ("initial_even", ctypes.c_int32),
         ("min_rate", ctypes.c_float),

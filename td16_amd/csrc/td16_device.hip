@@ -210,9 +210,7 @@ struct Ctx {
   int home;        // this wave's queue shard and feed shard
   uint32_t rng;    // LCG state for random shard probes
   bool drained;    // batched: this wave saw every feed shard exhausted
-  bool reorder;    // the request proved heavy: new frames visit greedy children first
-  bool gave;       // this item donated subtrees
-  uint64_t discard;  // nodes of this item expanded before a greedy restart
+  bool reorder;    // greedy order: frames visit the children gaining odd cycles first
 };
 
 enum { kExpandOk = 0, kExpandFound = 1 };
@@ -220,8 +218,8 @@ enum { kExpandOk = 0, kExpandFound = 1 };
 // Nodes after which an item counts as heavy: it gets raised wave priority
 // and, while unstarted requests remain, keeps up to kEarlyStock of its
 // subtrees queued.
-constexpr int kHeavyNodes = 256;
-constexpr int kEarlyStock = 512;
+constexpr int kHeavyNodes = 512;
+constexpr int kEarlyStock = 1024;
 
 // Packs the witness into the job state when this wave is the first finder.
 __device__ void record_witness(const Params &P, Ctx &c, WaveLds &w, int depth, int lastMove) {
@@ -492,7 +490,6 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   const uint64_t okBits = ballot(ok);
   const int cnt = __builtin_popcountll(okBits);
   if (cnt == 0) return 0;
-  c.gave = true;
   // account before publishing so the request cannot complete underneath;
   // the release fence below orders these before any consumer's claim
   if (lane == 0) {
@@ -615,6 +612,9 @@ __device__ void load_job(const Params &P, Ctx &c, WaveLds &w, uint32_t j, int li
   c.N = uni(job->N);
   c.n = c.N - 1;
   c.flags = uni(job->flags);
+  // FIRST searches take the greedy order unless the request asks for the
+  // lexicographic one; COUNT_ALL visits everything, in lexicographic order
+  c.reorder = (c.flags & (kFlagCountAll | kFlagLexOrder)) == 0;
   c.limit = limit;
   c.Rreg = lane < 20 ? (int)job->R[lane < 20 ? lane : 0] : 0;
   if (lane <= kMaxDepth) {
@@ -643,8 +643,6 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
   const int lane = lane_id();
   c.depth0 = depth0;
   c.expanded = 0;
-  c.discard = 0;
-  c.gave = false;
   c.solutions = 0;
   c.sinceCheck = 0;
   const bool countAll = (c.flags & kFlagCountAll) != 0;
@@ -690,26 +688,7 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
       // an item that has proved heavy gets the SIMD's issue priority over
       // the light requests sharing it, so it does not become the launch's
       // tail; light work fills the remaining issue slots
-      if (c.expanded >= (uint64_t)kHeavyNodes) {
-        __builtin_amdgcn_s_setprio(2);
-        // a FIRST search that proved heavy leaves the lexicographic order:
-        // this item restarts from its own root with frames that try the
-        // moves gaining two odd cycles first, and so does every item of the
-        // request opened from now on
-        if (!c.reorder && !countAll) {
-          c.reorder = true;
-          if (lane == 0) st_rlx(&st->reorder, 1u);
-          // restart only an item that has given nothing away, and count
-          // its nodes once: an exhaustive search then still expands (and
-          // reports) exactly the sequential search's node set
-          if (!c.gave) {
-            c.discard = c.expanded;
-            d = depth0;
-            if (expand(P, c, w, lex, d, uni64(w.frames[depth0].pi)) == kExpandFound) break;
-            continue;
-          }
-        }
-      }
+      if (c.expanded >= (uint64_t)kHeavyNodes) __builtin_amdgcn_s_setprio(2);
       const int want = lane0([&] {
         // balance = queued items - idle wavefronts
         int64_t bal = 0;
@@ -744,7 +723,7 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
 __device__ bool release_item(const Params &P, Ctx &c) {
   const int lane = lane_id();
   JobState *st = &P.states[c.job];
-  const uint64_t expanded = c.expanded - c.discard;
+  const uint64_t expanded = c.expanded;
   if (!c.shared) {
     // private request (root item, never donated from): this wave is the
     // only one that ever held it, so it finalizes with plain stores
@@ -780,15 +759,12 @@ __device__ __forceinline__ uint32_t result_word(const Params &P, uint32_t j, int
   else if (q == 16 || q == 17) src = q;       // t0
   if (src >= 0) return st[src];
   if (q == 13) return P.jobs[j].tag;
-  if (q == 15) return st[14];                 // reorder flag
   if (q == 18 || q == 19) {
     const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
     return q == 18 ? (uint32_t)t1 : (uint32_t)(t1 >> 32);
   }
   return 0u;  // seq, pad
 }
-static_assert(offsetof(JobState, reorder) == 56 && offsetof(DevResult, reordered) == 60,
-              "result_word copies the reorder flag by word");
 static_assert(offsetof(JobState, found) == 4 && offsetof(JobState, limit) == 8 &&
                   offsetof(JobState, split) == 12 && offsetof(JobState, expanded) == 16 &&
                   offsetof(JobState, depth) == 32 && offsetof(JobState, moves) == 36 &&
@@ -843,7 +819,6 @@ __device__ int start_job(const Params &P, uint32_t j) {
     st->expanded = 0;
     st->solutions = 0;
     st->depth = 0;
-    st->reorder = 0;
     st->t0 = __builtin_amdgcn_s_memrealtime();
   }
   // no fence: until this wave donates (which releases), no other wavefront
@@ -934,7 +909,6 @@ td16_search_kernel(Params P) {
   c.home = (int)((blockIdx.x * kWavesPerBlock + wid) % kQueueShards);
   c.rng = 0x9E3779B9u * (blockIdx.x * kWavesPerBlock + wid + 1);
   c.drained = false;
-  c.reorder = false;
   bool idle = false;
   int backoff = 1;
   for (;;) {
@@ -1006,20 +980,14 @@ td16_search_kernel(Params P) {
       pi = uni64(P.jobs[j].pi0);
       depth0 = 0;
       c.shared = false;
-      c.reorder = false;
     } else {
       j = (uint32_t)uni((int)it.job);
       c.job = j;
       c.shared = true;
       // a request that already has its witness: release the item unopened
       // (COUNT_ALL requests never set found)
-      const int flags_ = lane0([&] {
-        return (int)ld_rlx(&P.states[j].found) | ((int)ld_rlx(&P.states[j].reorder) << 1);
-      });
-      c.reorder = (flags_ & 2) != 0;
-      if ((flags_ & 1) != 0) {
+      if (lane0([&] { return (int)ld_rlx(&P.states[j].found); }) != 0) {
         c.expanded = 0;
-        c.discard = 0;
         c.solutions = 0;
         if (release_item(P, c)) publish_result(P, j);
         continue;
@@ -1052,7 +1020,6 @@ td16_search_kernel(Params P) {
       pi = uni64(P.jobs[j].pi0);
       depth0 = 0;
       c.shared = false;
-      c.reorder = lane0([&] { return (int)st->reorder; }) != 0;
     }
   }
 }

@@ -160,7 +160,7 @@ int make_job_checked(const td16_request *r, Job *job, int *trivial, char *msg, i
   job->maxDepth = (uint8_t)r->max_moves;
   job->flags = (uint8_t)((r->moveset == TD16_MOVES_ALL ? kFlagAllMoves : 0) |
                          (r->mode == TD16_COUNT_ALL ? kFlagCountAll : 0) |
-                         (r->iddfs ? kFlagIddfs : 0));
+                         (r->iddfs ? kFlagIddfs : 0) | (r->lex_order ? kFlagLexOrder : 0));
   if (r->iddfs) {
     const int odd = odd_cycles(r->omega, N);
     int b = 0;

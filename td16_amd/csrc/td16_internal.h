@@ -29,6 +29,7 @@ enum JobFlags : uint8_t {
   kFlagAllMoves = 1,   // every position triple (else the reference filter)
   kFlagCountAll = 2,   // no early exit
   kFlagIddfs = 4,      // iterative deepening on the depth limit
+  kFlagLexOrder = 8,   // FIRST search keeps the lexicographic order throughout
 };
 
 // One request as the kernel reads it (host-written, 128 B).
@@ -56,7 +57,7 @@ struct alignas(64) JobState {
   uint64_t solutions;
   int32_t depth;         // witness length
   uint16_t moves[10];    // witness, packed i | j<<5 | k<<10
-  uint32_t reorder;      // 1 once the FIRST search proved heavy and left the lexicographic order
+  uint8_t pad[4];
   uint64_t t0;           // device real-time counter (100 MHz) when the request started
   uint8_t pad2[56];
 };
@@ -73,7 +74,7 @@ struct alignas(16) DevResult {
   uint16_t moves[10];
   uint32_t tag;
   uint32_t seq;          // served loop: completion sequence + 1 once written
-  uint32_t reordered;    // the search left the lexicographic order
+  uint8_t pad[4];
   uint64_t t0, t1;       // device real-time counter at start and at publication
 };
 static_assert(sizeof(DevResult) == 80, "DevResult layout");

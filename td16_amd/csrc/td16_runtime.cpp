@@ -188,7 +188,7 @@ void decode(const DevResult &r, td16_result *out) {
   out->status = TD16_OK;
   out->found = r.found;
   out->depth = r.found ? r.depth : 0;
-  out->split = (r.split != 0 || r.reordered != 0) ? 1 : 0;
+  out->split = r.split != 0 ? 1 : 0;
   out->expanded = r.expanded;
   out->solutions = r.solutions;
   out->device_us = (r.t1 >= r.t0) ? (double)(r.t1 - r.t0) * 0.01 : 0.0;  // 100 MHz ticks

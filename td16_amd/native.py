@@ -26,7 +26,7 @@ def _u8(seq: Sequence[int]):
 
 def make_request(beta: Sequence[int], omega: Sequence[int], initial_even: int, min_rate: float,
                  max_moves: int, full_sorting: bool = True, moveset: int = MOVES_REF,
-                 mode: int = FIRST, iddfs: bool = False) -> Request:
+                 mode: int = FIRST, iddfs: bool = False, lex_order: bool = False) -> Request:
     if len(beta) != len(omega):
         raise Td16Error(1, "beta and omega must have equal lengths")
     if len(beta) > 17:
@@ -37,6 +37,7 @@ def make_request(beta: Sequence[int], omega: Sequence[int], initial_even: int, m
     for q, x in enumerate(omega):
         r.omega[q] = x & 0xFF
     r.n = len(beta)
+    r.lex_order = 1 if lex_order else 0
     r.initial_even = int(initial_even)
     r.min_rate = float(min_rate)
     r.max_moves = int(max_moves)
@@ -48,7 +49,7 @@ def make_request(beta: Sequence[int], omega: Sequence[int], initial_even: int, m
 
 
 def linear_request(perm: int, n: int, k: int, moveset: int = MOVES_REF, mode: int = FIRST,
-                   iddfs: bool = False) -> Request:
+                   iddfs: bool = False, lex_order: bool = False) -> Request:
     """A nibble-packed linear permutation as a request: beta = [pi, n],
     omega(y) = pred(y) + 1, min_rate by the emitter's convention."""
     r = Request()
@@ -56,6 +57,7 @@ def linear_request(perm: int, n: int, k: int, moveset: int = MOVES_REF, mode: in
                                    ctypes.byref(r))
     if st != 0:
         raise Td16Error(st, "invalid linear instance")
+    r.lex_order = 1 if lex_order else 0
     return r
 
 

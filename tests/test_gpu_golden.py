@@ -43,7 +43,8 @@ def assert_matches(O, native, req, res, ref, exact=False):
 def test_config1_iddfs_distances(native, exp):
     rows = exp["config1_iddfs"]
     for ms, col in ((native.MOVES_ALL, 1), (native.MOVES_REF, 2)):
-        reqs = [native.linear_request(r[0], 10, 9, ms, native.FIRST, True) for r in rows]
+        reqs = [native.linear_request(r[0], 10, 9, ms, native.FIRST, True, lex_order=True)
+                for r in rows]
         run = native.batch(reqs)
         for q, r in enumerate(rows):
             assert run.results[q].found == 1 and run.results[q].depth == r[col]
@@ -51,7 +52,8 @@ def test_config1_iddfs_distances(native, exp):
 
 def test_config2_first_witnesses(O, native, exp):
     ns, ks, perms = native.gen_products(1, 12, 16, 4, 9, 600)
-    reqs = [native.linear_request(int(perms[q]), int(ns[q]), int(ks[q])) for q in range(600)]
+    reqs = [native.linear_request(int(perms[q]), int(ns[q]), int(ks[q]), lex_order=True)
+            for q in range(600)]
     run = native.batch(reqs)
     for row in exp["config2_first"]:
         q = row["q"]
@@ -63,13 +65,13 @@ def test_config3_depth8_and_nonexistence_proofs(O, native, exp):
         inst = json.load(f)
     _, _, p3 = native.gen_products(2, 16, 16, 9, 9, 1999)
     perms = list(p3) + [inst["config3"]["reverse"]]
-    reqs = [native.linear_request(int(perms[q]), 16, 8) for q in range(150)]
+    reqs = [native.linear_request(int(perms[q]), 16, 8, lex_order=True) for q in range(150)]
     run = native.batch(reqs)
     for row in exp["config3_first"]:
         assert_matches(O, native, reqs[row["q"]], run.results[row["q"]], row)
     # complete depth-8 trees: node counts are exact however the work was shared
     proofs = exp["config3_nonexistence"]
-    reqs = [native.linear_request(p["perm"], 16, 8) for p in proofs]
+    reqs = [native.linear_request(p["perm"], 16, 8, lex_order=True) for p in proofs]
     for repeat in range(2):
         run = native.batch(reqs)
         for q, p in enumerate(proofs):
@@ -162,7 +164,8 @@ def test_non_full_sorting_against_oracle(O, native):
     run = native.batch(reqs)
     for q, req in enumerate(reqs):
         beta, omega = pair(req)
-        ref = O.search(beta, omega, req.initial_even, req.min_rate, req.max_moves, False)
+        ref = O.search(beta, omega, req.initial_even, req.min_rate, req.max_moves, False,
+                       greedy=True)
         r = run.results[q]
         assert r.found == ref.found
         if not r.split:

@@ -2,7 +2,9 @@
 
 Bars: found / not-found, witness replay validity, and -- for searches that a
 single wavefront ran from start to finish (result.split == 0) or that were
-exhaustive -- bit-exact witnesses and node counts.
+exhaustive -- bit-exact witnesses and node counts.  FIRST searches are
+compared with the oracle in the same visiting order: greedy by default,
+lexicographic (the reference's) when the request sets lex_order.
 """
 import math
 
@@ -25,14 +27,15 @@ def pair_of(req):
 
 def check_against_oracle(O, native, req, res, exact_counts=False):
     beta, omega = pair_of(req)
+    greedy = req.mode == 0 and not req.lex_order
     if req.iddfs:
-        d, ref = O.iddfs(beta, omega, req.moveset, req.max_moves)
+        d, ref = O.iddfs(beta, omega, req.moveset, req.max_moves, greedy=greedy)
         assert res.found == (d >= 0)
         if d >= 0:
             assert res.depth == d
     else:
         ref = O.search(beta, omega, req.initial_even, req.min_rate, req.max_moves,
-                       bool(req.full_sorting), req.moveset, req.mode)
+                       bool(req.full_sorting), req.moveset, req.mode, greedy=greedy)
         assert res.found == ref.found
     if res.found and res.depth > 0:
         sym = native.positions_to_symbols(beta, res.witness())
@@ -101,3 +104,29 @@ def test_batch_iddfs_all_n7(O, native):
     for q in range(len(reqs)):
         assert run.results[q].found == 1
         assert run.results[q].depth == dist[q + 1]
+
+
+def test_both_orders_against_oracle(O, native, slots):
+    """Searches of a few hundred nodes in both visiting orders: each matches
+    the oracle in its own order (exactly when one wavefront ran it), and the
+    greedy order needs no more nodes than the lexicographic one overall."""
+    ns, ks, perms = native.gen_products(1, 12, 16, 4, 9, 600)
+    picked = []
+    for q in range(len(ns)):
+        req = native.linear_request(int(perms[q]), int(ns[q]), int(ks[q]), lex_order=True)
+        beta, omega = pair_of(req)
+        ref = O.search(beta, omega, req.initial_even, req.min_rate, req.max_moves,
+                       bool(req.full_sorting), req.moveset, req.mode)
+        if ref.found and 40 <= ref.expanded <= 20000:
+            picked.append(req)
+        if len(picked) == 12:
+            break
+    assert picked
+    nodes = {0: 0, 1: 0}
+    for req in picked:
+        for lex in (1, 0):
+            req.lex_order = lex
+            res = native.search_ex(lex, req)
+            check_against_oracle(O, native, req, res)
+            nodes[lex] += res.expanded
+    assert nodes[0] <= nodes[1]

@@ -284,3 +284,91 @@ def test_generators_match_manifest(O):
     # SplitMix64 reference stream (seed 0): the first output of the published
     # algorithm is 0xe220a8397b1dcdaf
     assert O.splitmix64_stream(0, 1)[0] == 0xE220A8397B1DCDAF
+
+
+def greedy_search_py(beta, omega, max_depth, sat, reach):
+    """Pure-Python restatement of the greedy FIRST order for small inputs:
+    the reference filter (no fixed point, not three distinct cycles, delta 0
+    or +2) with deltas by full recount, children of a node evaluated in
+    lexicographic order (a satisfying one ends the search), expandable
+    children visited gaining ones first, each class in lexicographic order."""
+    n = len(beta)
+    state = {"expanded": 0, "witness": None}
+
+    def dfs(beta, omega, odd, depth, path):
+        state["expanded"] += 1
+        kids = []
+        for i, j, k in itertools.combinations(range(n), 3):
+            a, b, c = beta[i], beta[j], beta[k]
+            if omega[a] == a or omega[b] == b or omega[c] == c:
+                continue
+            lab = cycle_sets(omega)
+            if len({lab[a], lab[b], lab[c]}) == 3:
+                continue
+            child = recount_after(omega, a, b, c)
+            codd = odd_cycles_py(child)
+            if codd - odd not in (0, 2):
+                continue
+            if (sat[depth + 1] >> codd) & 1:
+                state["witness"] = path + [(i, j, k)]
+                return True
+            if depth + 1 < max_depth and (reach[depth + 1] >> codd) & 1:
+                kids.append((codd > odd, i, j, k, child, codd))
+        for gain in (True, False):
+            for g, i, j, k, child, codd in kids:
+                if g != gain:
+                    continue
+                cb = beta[:i] + beta[j:k] + beta[i:j] + beta[k:]
+                if dfs(cb, child, codd, depth + 1, path + [(i, j, k)]):
+                    return True
+        return False
+
+    found = dfs(list(beta), list(omega), odd_cycles_py(omega), 0, [])
+    return found, state["witness"], state["expanded"]
+
+
+def test_greedy_order_matches_python_restatement(O):
+    rng = random.Random(23)
+    checked = 0
+    for trial in range(120):
+        n = rng.randint(4, 7)
+        beta = list(range(n))
+        rng.shuffle(beta)
+        omega = list(range(n))
+        rng.shuffle(omega)
+        for k in (2, 3, 4):
+            odd = O.count_odd_cycles(omega)
+            sat, reach = O.rate_tables(n, odd, 0.0, k, True)
+            r = O.search(beta, omega, odd, 0.0, k, True, O.MOVES_REF, O.FIRST, greedy=True)
+            found, wit, expanded = greedy_search_py(beta, omega, k, sat, reach)
+            assert bool(r.found) == found
+            assert r.expanded == expanded
+            if found:
+                assert r.witness() == wit
+                checked += 1
+    assert checked > 20
+
+
+def test_greedy_order_is_order_independent_where_it_must_be(O):
+    # exhaustive searches (no witness, or COUNT_ALL) expand the same node set
+    # in either order; FIRST found / not-found never depends on the order
+    ns, ks, perms = O.gen_products(5, 8, 10, 3, 5, 80)
+    nonexist = 0
+    for q in range(len(ns)):
+        beta, omega = O.linear_to_pair(int(perms[q]), int(ns[q]))
+        odd = O.count_odd_cycles(omega)
+        k = int(ks[q])
+        for bound in (k, max(1, k - 2)):
+            lex = O.search(beta, omega, odd, 0.0, bound, True)
+            grd = O.search(beta, omega, odd, 0.0, bound, True, greedy=True)
+            assert lex.found == grd.found
+            if not lex.found:
+                nonexist += 1
+                assert lex.expanded == grd.expanded
+            else:
+                sym = O.positions_to_symbols(beta, grd.witness())
+                assert O.validate(beta, omega, sym, bound) == 0
+        ca = O.search(beta, omega, odd, 0.0, k, True, O.MOVES_REF, O.COUNT_ALL)
+        cg = O.search(beta, omega, odd, 0.0, k, True, O.MOVES_REF, O.COUNT_ALL, greedy=True)
+        assert (ca.expanded, ca.solutions) == (cg.expanded, cg.solutions)
+    assert nonexist > 10

@@ -186,7 +186,8 @@ struct WaveLds {
   uint32_t satTab[kMaxDepth + 1];
   uint32_t expTab[kMaxDepth + 1];
   uint16_t path[kMaxDepth + 1];  // path[d]: move applied at depth d
-  uint16_t pad[6];
+  uint16_t wit[kMaxDepth + 1];   // witness of a request no other wavefront touched
+  uint16_t pad[4];
 };
 
 struct BlockLds {
@@ -210,7 +211,15 @@ struct Ctx {
   int home;        // this wave's queue shard and feed shard
   uint32_t rng;    // LCG state for random shard probes
   bool drained;    // batched: this wave saw every feed shard exhausted
+  bool homeDone;   // batched: this wave saw its home feed shard exhausted
+  uint32_t published;  // batched: requests published and not yet counted globally
   bool reorder;    // greedy order: frames visit the children gaining odd cycles first
+  // request-level state (the request this wave started, or the item's)
+  uint32_t jw;       // lane q < 32: word q of the request's 128-byte record
+  bool everShared;   // some item of the request was donated
+  uint64_t t0;       // device clock when the request started
+  uint64_t totExpanded, totSolutions;  // finished items, while never shared
+  int found, wdepth; // witness kept in LDS (w.wit), while never shared
 };
 
 enum { kExpandOk = 0, kExpandFound = 1 };
@@ -226,11 +235,20 @@ __device__ void record_witness(const Params &P, Ctx &c, WaveLds &w, int depth, i
   JobState *st = &P.states[c.job];
   // the witness is published by the release on `pending` at the item's end;
   // a request no other wavefront has touched needs no atomic at all
+  const int lane = lane_id();
+  if (!c.everShared) {
+    // no other wavefront ever held this request: the witness stays in LDS
+    // and the result is published from there
+    if (lane < depth) w.wit[lane] = (lane == depth - 1) ? (uint16_t)lastMove : w.path[lane];
+    c.found = 1;
+    c.wdepth = depth;
+    wave_sync();
+    return;
+  }
   int won = 1;
   if (c.shared) won = lane0([&] { return cas_rlx(&st->found, 0, 1) ? 1 : 0; });
-  else if (lane_id() == 0) st->found = 1;
+  else if (lane == 0) st->found = 1;
   if (!won) return;
-  const int lane = lane_id();
   if (lane < depth) {
     const uint16_t mv = (lane == depth - 1) ? (uint16_t)lastMove : w.path[lane];
     st->moves[lane] = mv;
@@ -493,10 +511,18 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   // account before publishing so the request cannot complete underneath;
   // the release fence below orders these before any consumer's claim
   if (lane == 0) {
-    add_rlx(&P.states[c.job].pending, cnt);
-    add_rlx(&P.states[c.job].split, cnt);
+    JobState *st = &P.states[c.job];
+    if (!c.everShared) {
+      // first donation: the counts of finished IDDFS rounds, kept in
+      // registers so far, join the shared state (published by the fence)
+      st->expanded = c.totExpanded;
+      st->solutions = c.totSolutions;
+    }
+    add_rlx(&st->pending, cnt);
+    add_rlx(&st->split, cnt);
     add_rlx(&P.ctrl->balance[c.home % kBalanceShards].v, (int64_t)cnt);
   }
+  c.everShared = true;
   const uint64_t slot = (uint64_t)shard * cap + pos % cap;
   if (ok) {
     const int mv = frame_move_lane(F, fhdr, lex, ch, lane);
@@ -570,16 +596,21 @@ __device__ int claim_job(const Params &P, Ctx &c) {
     const uint64_t e = (uint64_t)(s + 1) * per;
     return e < nj ? (uint32_t)e : nj;
   };
+  // one atomic claims a request; the counter of a shard without requests is
+  // never touched (the host does not reset it)
   auto take = [&](int s) {
     return lane0([&] {
-      const uint32_t end = shard_end((uint32_t)s);
-      if ((uint64_t)s * per >= nj || ld_rlx(&ctl->feed[s].next) >= end) return -1;
+      if ((uint64_t)s * per >= nj) return -1;
       const uint32_t g = add_rlx(&ctl->feed[s].next, 1u);
-      return g < end ? (int)g : -1;
+      return g < shard_end((uint32_t)s) ? (int)g : -1;
     });
   };
-  int got = take(c.home);
-  if (got >= 0) return got;
+  int got = -1;
+  if (!c.homeDone) {
+    got = take(c.home);
+    if (got >= 0) return got;
+    c.homeDone = true;
+  }
   if (lane0([&] { return (int)ld_rlx(&ctl->drained); }) != 0) {
     c.drained = true;
     return -1;
@@ -604,19 +635,47 @@ __device__ int claim_job(const Params &P, Ctx &c) {
   }
 }
 
-// Loads the request's parameters into the wave context and LDS tables.
+// Lane q < 32 reads word q of request j's 128-byte record: one coalesced
+// load brings everything the search needs into the wave.
+__device__ __forceinline__ uint32_t job_word(const Params &P, uint32_t j) {
+  const int lane = lane_id();
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(&P.jobs[j]);
+  return lane < 32 ? src[lane] : 0u;
+}
+static_assert(offsetof(Job, R) == 8 && offsetof(Job, N) == 28 && offsetof(Job, maxDepth) == 29 &&
+                  offsetof(Job, flags) == 30 && offsetof(Job, firstLimit) == 31 &&
+                  offsetof(Job, sat) == 32 && offsetof(Job, reach) == 72 &&
+                  offsetof(Job, tag) == 124,
+              "the record is read by word");
+// word 7: N | maxDepth << 8 | flags << 16 | firstLimit << 24
+__device__ __forceinline__ uint32_t job_w7(const Ctx &c) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)c.jw, 7);
+}
+__device__ __forceinline__ uint64_t job_pi0(const Ctx &c) {
+  return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)c.jw, 1) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)c.jw, 0);
+}
+__device__ __forceinline__ int job_max_depth(const Ctx &c) { return (int)((job_w7(c) >> 8) & 0xFFu); }
+
+// Loads the request's parameters (from c.jw) into the wave context and LDS
+// tables.
 __device__ void load_job(const Params &P, Ctx &c, WaveLds &w, uint32_t j, int limit) {
   const int lane = lane_id();
-  const Job *job = &P.jobs[j];
   c.job = j;
-  c.N = uni(job->N);
+  const uint32_t w7 = job_w7(c);
+  c.N = (int)(w7 & 0xFFu);
   c.n = c.N - 1;
-  c.flags = uni(job->flags);
+  c.flags = (int)((w7 >> 16) & 0xFFu);
   // FIRST searches take the greedy order unless the request asks for the
   // lexicographic one; COUNT_ALL visits everything, in lexicographic order
   c.reorder = (c.flags & (kFlagCountAll | kFlagLexOrder)) == 0;
   c.limit = limit;
-  c.Rreg = lane < 20 ? (int)job->R[lane < 20 ? lane : 0] : 0;
+  // R[y] is byte 8 + y of the record, sat[d] word 8 + d, reach[d] word 18 + d
+  const uint32_t rword = (uint32_t)bperm(lane < 20 ? 2 + (lane >> 2) : 2, (int)c.jw);
+  c.Rreg = lane < 20 ? (int)((rword >> (8 * (lane & 3))) & 0xFFu) : 0;
+  const int tl = lane <= kMaxDepth ? lane : 0;
+  const uint32_t satw = (uint32_t)bperm(8 + tl, (int)c.jw);
+  const uint32_t reachw = (uint32_t)bperm(18 + tl, (int)c.jw);
   if (lane <= kMaxDepth) {
     uint32_t s, r;
     if (c.flags & kFlagIddfs) {
@@ -625,8 +684,8 @@ __device__ void load_job(const Params &P, Ctx &c, WaveLds &w, uint32_t j, int li
       const int lo = c.N - 2 * left;
       r = left < 0 ? 0u : (lo <= 0 ? ((2u << c.N) - 1u) : (((2u << c.N) - 1u) & ~((1u << lo) - 1u)));
     } else {
-      s = job->sat[lane];
-      r = job->reach[lane];
+      s = satw;
+      r = reachw;
     }
     w.satTab[lane] = s;
     w.expTab[lane] = (lane < limit) ? r : 0u;
@@ -725,8 +784,14 @@ __device__ bool release_item(const Params &P, Ctx &c) {
   JobState *st = &P.states[c.job];
   const uint64_t expanded = c.expanded;
   if (!c.shared) {
-    // private request (root item, never donated from): this wave is the
-    // only one that ever held it, so it finalizes with plain stores
+    // a request no other wavefront ever held keeps its counts in registers
+    if (!c.everShared) {
+      c.totExpanded += expanded;
+      c.totSolutions += c.solutions;
+      return true;
+    }
+    // root item of a later IDDFS round of a once-shared request: this wave
+    // is the only one holding it, so it finalizes with plain stores
     if (lane == 0) {
       st->expanded += expanded;
       st->solutions += c.solutions;
@@ -747,9 +812,33 @@ __device__ bool release_item(const Params &P, Ctx &c) {
   return last != 0;
 }
 
-// Word q of the 80-byte DevResult record, written by lane q (q < 20) from the
-// request state; the caller has already acquired what other wavefronts wrote.
-__device__ __forceinline__ uint32_t result_word(const Params &P, uint32_t j, int q) {
+// Word q of the 80-byte DevResult record, written by lane q (q < 20).  A
+// request no other wavefront ever held is described by the wave's own
+// registers and LDS; otherwise by its state in HBM (the caller has already
+// acquired what other wavefronts wrote).
+__device__ __forceinline__ uint32_t result_word(const Params &P, const Ctx &c, const WaveLds &w,
+                                                uint32_t j, int q) {
+  if (q == 13) return (uint32_t)__builtin_amdgcn_readlane((int)c.jw, 31);  // tag
+  if (q == 18 || q == 19) {
+    const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
+    return q == 18 ? (uint32_t)t1 : (uint32_t)(t1 >> 32);
+  }
+  if (!c.everShared) {
+    switch (q) {
+      case 0: return (uint32_t)c.found;
+      case 1: return c.found ? (uint32_t)c.wdepth : 0u;
+      case 3: return (uint32_t)c.limit;
+      case 4: return (uint32_t)c.totExpanded;
+      case 5: return (uint32_t)(c.totExpanded >> 32);
+      case 6: return (uint32_t)c.totSolutions;
+      case 7: return (uint32_t)(c.totSolutions >> 32);
+      case 16: return (uint32_t)c.t0;
+      case 17: return (uint32_t)(c.t0 >> 32);
+      default: break;
+    }
+    if (q >= 8 && q < 13) return reinterpret_cast<const uint32_t *>(w.wit)[q - 8];
+    return 0u;  // split, seq, pad
+  }
   const uint32_t *st = reinterpret_cast<const uint32_t *>(&P.states[j]);
   // found depth split limit | expanded solutions | moves[10] | tag seq pad | t0 t1
   int src = -1;
@@ -758,13 +847,9 @@ __device__ __forceinline__ uint32_t result_word(const Params &P, uint32_t j, int
   else if (q < 13) src = q + 1;               // moves
   else if (q == 16 || q == 17) src = q;       // t0
   if (src >= 0) return st[src];
-  if (q == 13) return P.jobs[j].tag;
-  if (q == 18 || q == 19) {
-    const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
-    return q == 18 ? (uint32_t)t1 : (uint32_t)(t1 >> 32);
-  }
   return 0u;  // seq, pad
 }
+static_assert(offsetof(WaveLds, wit) % 4 == 0, "the witness is read by word");
 static_assert(offsetof(JobState, found) == 4 && offsetof(JobState, limit) == 8 &&
                   offsetof(JobState, split) == 12 && offsetof(JobState, expanded) == 16 &&
                   offsetof(JobState, depth) == 32 && offsetof(JobState, moves) == 36 &&
@@ -774,24 +859,17 @@ static_assert(offsetof(DevResult, moves) == 32 && offsetof(DevResult, tag) == 52
                   offsetof(DevResult, t0) == 64,
               "result_word writes DevResult by word");
 
-__device__ void publish_result(const Params &P, uint32_t j) {
+__device__ void publish_result(const Params &P, Ctx &c, const WaveLds &w, uint32_t j) {
   const int lane = lane_id();
   if (!P.served) {
     // the host reads batched results after the kernel has ended
-    if (lane < 20) reinterpret_cast<uint32_t *>(&P.results[j])[lane] = result_word(P, j, lane);
-    if (lane == 0) {
-      // feed shard of request j; its last publisher counts the shard done
-      Control *ctl = P.ctrl;
-      const uint32_t per = ctl->perShard;
-      const uint32_t s = j / per;
-      const uint32_t lo = s * per;
-      const uint32_t hi = ctl->numJobs - lo < per ? ctl->numJobs : lo + per;
-      if (add_rlx(&ctl->feed[s].done, 1u) + 1u == hi - lo) add_rlx(&ctl->shardsDone, 1u);
-    }
+    if (lane < 20) reinterpret_cast<uint32_t *>(&P.results[j])[lane] = result_word(P, c, w, j, lane);
+    c.published++;  // added to the shared count once this wave runs out of work
   } else {
     // record into the host ring slot of job slot j, then its index into the
     // completion ring; the system fence orders the record before the index
-    if (lane < 20) reinterpret_cast<uint32_t *>(&P.hostResults[j])[lane] = result_word(P, j, lane);
+    if (lane < 20)
+      reinterpret_cast<uint32_t *>(&P.hostResults[j])[lane] = result_word(P, c, w, j, lane);
     __threadfence_system();
     if (lane == 0) {
       const uint64_t cseq = add_rlx(&P.ctrl->completions, (uint64_t)1);
@@ -806,11 +884,18 @@ __device__ void publish_result(const Params &P, uint32_t j) {
 }
 
 // Starts request j (root item) in this wave; returns its first depth limit.
-__device__ int start_job(const Params &P, uint32_t j) {
+__device__ int start_job(const Params &P, Ctx &c, uint32_t j) {
   const int lane = lane_id();
-  const Job *job = &P.jobs[j];
-  const int limit = uni((job->flags & kFlagIddfs) ? job->firstLimit : job->maxDepth);
+  const uint32_t w7 = job_w7(c);
+  const int limit = (int)((((w7 >> 16) & kFlagIddfs) ? (w7 >> 24) : (w7 >> 8)) & 0xFFu);
+  c.t0 = __builtin_amdgcn_s_memrealtime();
+  c.everShared = false;
+  c.totExpanded = 0;
+  c.totSolutions = 0;
+  c.found = 0;
+  c.wdepth = 0;
   if (lane == 0) {
+    // read by other wavefronts only once this wave donates
     JobState *st = &P.states[j];
     st->pending = 1;
     st->found = 0;
@@ -819,7 +904,7 @@ __device__ int start_job(const Params &P, uint32_t j) {
     st->expanded = 0;
     st->solutions = 0;
     st->depth = 0;
-    st->t0 = __builtin_amdgcn_s_memrealtime();
+    st->t0 = c.t0;
   }
   // no fence: until this wave donates (which releases), no other wavefront
   // reads this state
@@ -890,10 +975,10 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut) {
 
 }  // namespace
 
-// Register budget for five waves per SIMD: the code the cap pushes to
-// scratch is per-request set-up, not the per-node path, and the measured
-// mix runs fastest with it at three resident waves per SIMD.
-__global__ void __launch_bounds__(kBlockThreads, 5)
+// Register budget for four waves per SIMD (128 VGPRs): the few registers the
+// cap pushes to scratch belong to per-request set-up, not the per-node path,
+// and the measured mix runs fastest with it at three resident waves per SIMD.
+__global__ void __launch_bounds__(kBlockThreads, 4)
 td16_search_kernel(Params P) {
   __shared__ BlockLds lds;
   for (int q = threadIdx.x; q < kLexEntries; q += blockDim.x) lds.lex[q] = P.lex[q];
@@ -909,6 +994,8 @@ td16_search_kernel(Params P) {
   c.home = (int)((blockIdx.x * kWavesPerBlock + wid) % kQueueShards);
   c.rng = 0x9E3779B9u * (blockIdx.x * kWavesPerBlock + wid + 1);
   c.drained = false;
+  c.homeDone = false;
+  c.published = 0;
   bool idle = false;
   int backoff = 1;
   for (;;) {
@@ -946,7 +1033,14 @@ td16_search_kernel(Params P) {
       }
       int done = 0;
       if (!P.served) {
-        done = lane0([&] { return ld_rlx(&P.ctrl->shardsDone) >= (uint32_t)kFeedShards ? 1 : 0; });
+        done = lane0([&] {
+          if (c.published) add_rlx(&P.ctrl->published[c.home % kBalanceShards].v, (int64_t)c.published);
+          int64_t sum = 0;
+#pragma unroll
+          for (int q = 0; q < kBalanceShards; ++q) sum += ld_rlx(&P.ctrl->published[q].v);
+          return sum >= (int64_t)P.ctrl->numJobs ? 1 : 0;
+        });
+        c.published = 0;
       } else {
         // device-side mirrors only: host memory is read by the rate-limited
         // poll in take_submitted, never by every idle wavefront
@@ -976,20 +1070,23 @@ td16_search_kernel(Params P) {
     uint64_t pi;
     int depth0, limit;
     if (haveJob) {
-      limit = start_job(P, j);
-      pi = uni64(P.jobs[j].pi0);
+      c.jw = job_word(P, j);
+      limit = start_job(P, c, j);
+      pi = job_pi0(c);
       depth0 = 0;
       c.shared = false;
     } else {
       j = (uint32_t)uni((int)it.job);
       c.job = j;
       c.shared = true;
+      c.everShared = true;
+      c.jw = job_word(P, j);
       // a request that already has its witness: release the item unopened
       // (COUNT_ALL requests never set found)
       if (lane0([&] { return (int)ld_rlx(&P.states[j].found); }) != 0) {
         c.expanded = 0;
         c.solutions = 0;
-        if (release_item(P, c)) publish_result(P, j);
+        if (release_item(P, c)) publish_result(P, c, w, j);
         continue;
       }
       pi = uni64(it.pi);
@@ -1001,23 +1098,33 @@ td16_search_kernel(Params P) {
       if (!run_item(P, c, w, lex, pi, depth0)) break;
       // last item of request j finished: next IDDFS limit or publish
       JobState *st = &P.states[j];
-      const int next = lane0([&] {
-        const Job *job = &P.jobs[j];
-        if ((job->flags & kFlagIddfs) && st->found == 0 && st->limit < job->maxDepth) {
-          const int nl = st->limit + 1;
-          st->limit = nl;
-          st->pending = 1;
-          return nl;
+      int next = 0;
+      if (!c.everShared) {
+        if ((c.flags & kFlagIddfs) && !c.found && c.limit < job_max_depth(c)) {
+          next = c.limit + 1;
+          if (lane == 0) {
+            st->limit = next;
+            st->pending = 1;
+          }
         }
-        return 0;
-      });
+      } else {
+        next = lane0([&] {
+          if ((c.flags & kFlagIddfs) && st->found == 0 && st->limit < job_max_depth(c)) {
+            const int nl = st->limit + 1;
+            st->limit = nl;
+            st->pending = 1;
+            return nl;
+          }
+          return 0;
+        });
+      }
       wave_sync();
       if (next == 0) {
-        publish_result(P, j);
+        publish_result(P, c, w, j);
         break;
       }
       load_job(P, c, w, j, next);
-      pi = uni64(P.jobs[j].pi0);
+      pi = job_pi0(c);
       depth0 = 0;
       c.shared = false;
     }

@@ -116,8 +116,6 @@ static_assert(kFeedShards == kQueueShards, "a wave's home shard indexes both");
 struct alignas(256) FeedShard {
   uint32_t next;         // next unclaimed request
   uint32_t pad0[63];
-  uint32_t done;         // requests published
-  uint32_t pad1[63];
 };
 
 // Donation demand (queued items minus idle wavefronts), kept as the sum of
@@ -137,9 +135,10 @@ struct alignas(256) Control {
   uint32_t pad0[62];
   uint32_t drained;      // batched: 1 once some wavefront saw every feed shard exhausted
   uint32_t pad1[63];
-  uint32_t shardsDone;   // batched: feed shards whose requests are all published
-  uint32_t pad2[63];
   Balance balance[kBalanceShards];
+  // batched: requests published, summed over the shards; each wavefront
+  // counts its own in a register and adds them when it runs out of work
+  Balance published[kBalanceShards];
   uint32_t jobsDone;     // served: requests published
   uint32_t pad4[63];
   // served loop

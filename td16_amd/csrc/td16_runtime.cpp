@@ -155,10 +155,8 @@ size_t init_feed(Control *c, uint32_t numJobs) {
   for (int s = 0; s < kFeedShards; ++s) {
     const uint64_t lo = std::min<uint64_t>((uint64_t)s * per, numJobs);
     c->feed[s].next = (uint32_t)lo;
-    c->feed[s].done = 0;
     if (lo < std::min<uint64_t>((uint64_t)(s + 1) * per, numJobs)) ++used;
   }
-  c->shardsDone = (uint32_t)kFeedShards - used;  // empty shards count as done
   // nonempty shards are the leading ones
   return offsetof(Control, feed) + (size_t)used * sizeof(FeedShard);
 }

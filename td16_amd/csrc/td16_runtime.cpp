@@ -90,7 +90,6 @@ struct Buffers {
   uint32_t jobsCap = 0;
   Job *dJobs = nullptr;
   JobState *dStates = nullptr;
-  DevResult *dResults = nullptr;
   Control *dCtrl = nullptr;
   QueueState *dQueue = nullptr;
   Item *dItems = nullptr;
@@ -98,13 +97,15 @@ struct Buffers {
   uint64_t qcap = 0;
   Control *hCtrl = nullptr;      // pinned
   Job *hJobs = nullptr;          // pinned
-  DevResult *hResults = nullptr; // pinned
+  // results live in coherent pinned host memory: the kernel writes each one
+  // over the bus as the request finishes, so no copy follows the launch
+  DevResult *hResults = nullptr;
+  DevResult *dResults = nullptr; // device view of hResults
 
   void release() {
     if (device >= 0) hipSetDevice(device);
     hipFree(dJobs);
     hipFree(dStates);
-    hipFree(dResults);
     hipFree(dCtrl);
     hipFree(dQueue);
     hipFree(dItems);
@@ -128,14 +129,15 @@ int alloc_buffers(Buffers &b, int device, uint32_t jobsCap, int64_t queueBytes, 
   const uint64_t slots = qcap * kQueueShards;
   TD16_HIP(hipMalloc(&b.dJobs, sizeof(Job) * jobsCap));
   TD16_HIP(hipMalloc(&b.dStates, sizeof(JobState) * jobsCap));
-  TD16_HIP(hipMalloc(&b.dResults, sizeof(DevResult) * jobsCap));
   TD16_HIP(hipMalloc(&b.dCtrl, sizeof(Control)));
   TD16_HIP(hipMalloc(&b.dQueue, sizeof(QueueState) * kQueueShards));
   TD16_HIP(hipMalloc(&b.dItems, sizeof(Item) * slots));
   TD16_HIP(hipMalloc(&b.dSeq, sizeof(uint64_t) * slots));
   TD16_HIP(hipHostMalloc(&b.hCtrl, sizeof(Control), hipHostMallocDefault));
   TD16_HIP(hipHostMalloc(&b.hJobs, sizeof(Job) * jobsCap, hipHostMallocDefault));
-  TD16_HIP(hipHostMalloc(&b.hResults, sizeof(DevResult) * jobsCap, hipHostMallocDefault));
+  TD16_HIP(hipHostMalloc(&b.hResults, sizeof(DevResult) * jobsCap,
+                         hipHostMallocCoherent | hipHostMallocMapped));
+  TD16_HIP(hipHostGetDevicePointer((void **)&b.dResults, b.hResults, 0));
   std::memset(b.hCtrl, 0, sizeof(Control));
   // queue counters start at zero, slot q expects producer position q
   static const QueueState zeroQueue[kQueueShards] = {};
@@ -253,8 +255,6 @@ int run_plain(Slot &s, const td16_request *req, td16_result *res, char *msg, int
                           s.stream));
   Params P = make_params(s.buf, s.dev);
   TD16_HIP((hipError_t)launch_search(P, s.blocks, s.stream));
-  TD16_HIP(hipMemcpyAsync(s.buf.hResults, s.buf.dResults, sizeof(DevResult),
-                          hipMemcpyDeviceToHost, s.stream));
   TD16_HIP(hipEventRecord(s.done, s.stream));
   TD16_HIP(hipEventSynchronize(s.done));
   decode(s.buf.hResults[0], res);
@@ -691,8 +691,6 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
     else
       TD16_HIP((hipError_t)launch_search(P, blocks, gBatch.stream));
     TD16_HIP(hipEventRecord(gBatch.ev1, gBatch.stream));
-    TD16_HIP(hipMemcpyAsync(b.hResults, b.dResults, sizeof(DevResult) * nj,
-                            hipMemcpyDeviceToHost, gBatch.stream));
     TD16_HIP(hipStreamSynchronize(gBatch.stream));
     const auto w1 = std::chrono::steady_clock::now();
     float ms = 0.f;

@@ -167,7 +167,7 @@ int td16_search_ex(int slot, const td16_request *req, td16_result *res,
 
 typedef struct {
     int32_t device;          /* HIP device ordinal */
-    int32_t waves_per_cu;    /* resident wavefronts per CU; 0 = 12 (tuned) */
+    int32_t waves_per_cu;    /* resident wavefronts per CU; 0 = 16 (tuned) */
     int32_t split_depth;     /* frames deeper than max_moves - split_depth are
                                 never donated; 0 = default (5) */
     int32_t donate_max;      /* children donated per donation; 0 = default (64) */

@@ -213,6 +213,12 @@ struct Ctx {
   bool drained;    // batched: this wave saw every feed shard exhausted
   bool homeDone;   // batched: this wave saw its home feed shard exhausted
   uint32_t published;  // batched: requests published and not yet counted globally
+  // batched look-ahead, taken while a request's record loads: the next
+  // request claimed from the home shard (lane 0, read when needed) and
+  // whether items were queued
+  int preG;
+  bool preValid;
+  int stockHint;   // -1 unknown, else 0/1
   bool reorder;    // greedy order: frames visit the children gaining odd cycles first
   // request-level state (the request this wave started, or the item's)
   uint32_t jw;       // lane q < 32: word q of the request's 128-byte record
@@ -582,6 +588,16 @@ __device__ int pop_item(const Params &P, Ctx &c, Item &it, WaveLds &w) {
 // ---------------------------------------------------------------------------
 // Requests
 // ---------------------------------------------------------------------------
+
+// End of feed shard s's request range (host-written size fields).
+__device__ __forceinline__ uint32_t feed_end(const Params &P, int s) {
+  const uint32_t per = P.ctrl->perShard, nj = P.ctrl->numJobs;
+  const uint64_t e = (uint64_t)(s + 1) * per;
+  return e < nj ? (uint32_t)e : nj;
+}
+
+// Requests at the end of a feed shard are not claimed ahead.
+constexpr int kLookAheadMargin = 96;
 
 // Batched feed: claims the next unstarted request, from the home feed shard
 // first, then from whichever other shard still has some (every lane inspects
@@ -996,6 +1012,8 @@ td16_search_kernel(Params P) {
   c.drained = false;
   c.homeDone = false;
   c.published = 0;
+  c.preValid = false;
+  c.stockHint = -1;
   bool idle = false;
   int backoff = 1;
   for (;;) {
@@ -1006,18 +1024,30 @@ td16_search_kernel(Params P) {
     // spares the probes of an empty queue.
     bool tryPop = true;
     if (!P.served && !c.drained) {
-      tryPop = lane0([&] {
-        int64_t bal = 0;
+      if (c.stockHint >= 0) {
+        tryPop = c.stockHint != 0;  // read while the last request's record loaded
+      } else {
+        tryPop = lane0([&] {
+          int64_t bal = 0;
 #pragma unroll
-        for (int q = 0; q < kBalanceShards; ++q) bal += ld_rlx(&P.ctrl->balance[q].v);
-        return bal > 0 ? 1 : 0;
-      }) != 0;
+          for (int q = 0; q < kBalanceShards; ++q) bal += ld_rlx(&P.ctrl->balance[q].v);
+          return bal > 0 ? 1 : 0;
+        }) != 0;
+      }
     }
+    c.stockHint = -1;
     bool haveItem = tryPop && pop_item(P, c, it, w) != 0;
     bool haveJob = false;
     if (!haveItem) {
       if (!P.served) {
-        const int jj = claim_job(P, c);
+        int jj = -1;
+        if (c.preValid) {
+          c.preValid = false;
+          const int g = uni(c.preG);
+          if (g < (int)feed_end(P, c.home)) jj = g;
+          else c.homeDone = true;
+        }
+        if (jj < 0) jj = claim_job(P, c);
         if (jj >= 0) {
           j = (uint32_t)jj;
           haveJob = true;
@@ -1070,7 +1100,28 @@ td16_search_kernel(Params P) {
     uint64_t pi;
     int depth0, limit;
     if (haveJob) {
+      // look ahead while this request's record loads: claim the home shard's
+      // next request (unless this one is among the shard's last, where a
+      // held request could wait behind a long search) and read the balance
+      // counters; one round trip covers all three
+      int64_t balv = 0;
+      if (!P.served && !c.drained) {
+        if (!c.homeDone && (int)j + kLookAheadMargin < (int)feed_end(P, c.home) &&
+            j >= (uint32_t)c.home * P.ctrl->perShard) {
+          if (lane == 0) c.preG = (int)add_rlx(&P.ctrl->feed[c.home].next, 1u);
+          c.preValid = true;
+        }
+        if (lane < kBalanceShards) balv = ld_rlx(&P.ctrl->balance[lane].v);
+      }
       c.jw = job_word(P, j);
+      if (!P.served && !c.drained) {
+        int64_t bal = 0;
+#pragma unroll
+        for (int q = 0; q < kBalanceShards; ++q)
+          bal += (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(balv >> 32), q) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)balv, q));
+        c.stockHint = bal > 0 ? 1 : 0;
+      }
       limit = start_job(P, c, j);
       pi = job_pi0(c);
       depth0 = 0;

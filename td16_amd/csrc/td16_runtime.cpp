@@ -599,7 +599,10 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
   const uint32_t nj = (uint32_t)jobs.size();
   // Feed order: longest expected search first (move bound, then symbols).
   // Heavy requests then start with the launch and are split once the feed
-  // drains, instead of starting late and becoming the launch's tail.
+  // drains, instead of starting late and becoming the launch's tail.  The
+  // sorted list is dealt round-robin to the feed shards (contiguous ranges
+  // of the table, see init_feed), so every shard runs its own
+  // longest-first sequence.
   {
     std::vector<uint32_t> order(nj);
     for (uint32_t q = 0; q < nj; ++q) order[q] = q;
@@ -607,14 +610,22 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
       const Job &x = jobs[a], &y = jobs[b];
       return x.maxDepth != y.maxDepth ? x.maxDepth > y.maxDepth : x.N > y.N;
     });
-    std::vector<Job> sortedJobs(nj);
-    std::vector<int> sortedWhere(nj);
-    for (uint32_t q = 0; q < nj; ++q) {
-      sortedJobs[q] = jobs[order[q]];
-      sortedWhere[q] = where[order[q]];
+    const uint32_t per = (nj + kFeedShards - 1) / kFeedShards;
+    std::vector<Job> dealtJobs(nj);
+    std::vector<int> dealtWhere(nj);
+    uint32_t next = 0;
+    for (uint32_t t = 0; next < nj; ++t) {
+      for (uint32_t s = 0; s < (uint32_t)kFeedShards && next < nj; ++s) {
+        const uint64_t pos = (uint64_t)s * per + t;
+        const uint64_t end = std::min<uint64_t>((uint64_t)(s + 1) * per, nj);
+        if (pos >= end) continue;
+        dealtJobs[pos] = jobs[order[next]];
+        dealtWhere[pos] = where[order[next]];
+        ++next;
+      }
     }
-    jobs.swap(sortedJobs);
-    where.swap(sortedWhere);
+    jobs.swap(dealtJobs);
+    where.swap(dealtWhere);
   }
 
   std::lock_guard<std::mutex> lk(gBatchMu);

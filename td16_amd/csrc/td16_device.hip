@@ -233,7 +233,7 @@ enum { kExpandOk = 0, kExpandFound = 1 };
 // Nodes after which an item counts as heavy: it gets raised wave priority
 // and, while unstarted requests remain, keeps up to kEarlyStock of its
 // subtrees queued.
-constexpr int kHeavyNodes = 512;
+constexpr int kHeavyNodes = 64;
 constexpr int kEarlyStock = 1024;
 
 // Packs the witness into the job state when this wave is the first finder.

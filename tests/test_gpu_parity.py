@@ -130,3 +130,31 @@ def test_both_orders_against_oracle(O, native, slots):
             check_against_oracle(O, native, req, res)
             nodes[lex] += res.expanded
     assert nodes[0] <= nodes[1]
+
+
+@pytest.mark.parametrize("count", [1, 2, 63, 64, 65, 129, 1000])
+def test_batch_sizes_around_feed_shards(O, native, count):
+    """Batches smaller than, equal to and just above the 64 feed shards (empty
+    and short shards, look-ahead claims near shard ends), mixed with requests
+    the host answers itself (n < 3, sorted IDDFS): every result belongs to its
+    own request."""
+    ns, ks, perms = native.gen_products(31 + count, 8, 14, 3, 7, count)
+    reqs = []
+    for q in range(count):
+        if q % 17 == 5:
+            reqs.append(native.make_request([0, 1], [1, 0], 0, 1.0, 3))       # n < 3
+        elif q % 17 == 11:
+            reqs.append(native.make_request(list(range(6)), list(range(6)), 6, 0.0, 5,
+                                            iddfs=True))                      # sorted
+        else:
+            reqs.append(native.linear_request(int(perms[q]), int(ns[q]), int(ks[q])))
+    run = native.batch(reqs)
+    for q, req in enumerate(reqs):
+        r = run.results[q]
+        assert r.status == 0
+        if q % 17 == 5:
+            assert r.found == 0 and r.depth == 0
+        elif q % 17 == 11:
+            assert r.found == 1 and r.depth == 0
+        else:
+            check_against_oracle(O, native, req, r)

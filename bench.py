@@ -26,6 +26,8 @@ import sys
 import threading
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -49,25 +51,50 @@ def build_requests(native, ns, ks, perms, moveset=0, mode=0):
     return arr
 
 
+def host_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 def verify(native, reqs, results):
-    """Replays every witness with the product's checker; returns (found, verified)."""
-    found = verified = 0
-    for q in range(len(results)):
-        r = results[q]
-        if r.status != 0:
-            continue
-        if r.found:
-            found += 1
-            req = reqs[q]
-            beta = list(req.beta[:req.n])
-            omega = list(req.omega[:req.n])
-            try:
-                sym = native.positions_to_symbols(beta, r.witness())
-                native.validate(beta, omega, sym, req.max_moves)
-                verified += 1
-            except Exception as e:  # counted as unverified
-                log("unverified witness", q, e)
+    """Replays every witness with the product's batched checker
+    (td16_certify_batch: positions -> symbols -> replay -> codec records);
+    returns (found, verified)."""
+    certs = native.certify_batch(reqs, results, host_threads())
+    found = sum(1 for q in range(len(results)) if results[q].status == 0 and results[q].found)
+    verified = int((certs.status == 0).sum())
+    for q in np.nonzero(certs.status < 0)[0][:5]:
+        log("unverified witness", int(q), int(certs.status[q]))
     return found, verified
+
+
+def certificate_rates(native, reqs, results, repeats=5):
+    """Host throughput of the certificate path over one batch's results:
+    certify (the emitter's replay + encode) and check (the checker's decode +
+    replay of stored values), best of `repeats`, on host_threads() threads."""
+    count = len(results)
+    betas = np.zeros((count, 17), np.uint8)
+    omegas = np.zeros((count, 17), np.uint8)
+    ns = np.zeros(count, np.int32)
+    mm = np.zeros(count, np.int32)
+    for q in range(count):
+        n = reqs[q].n
+        ns[q], mm[q] = n, reqs[q].max_moves
+        betas[q, :n] = reqs[q].beta[:n]
+        omegas[q, :n] = reqs[q].omega[:n]
+    th = host_threads()
+    best_c = best_k = float("inf")
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        certs = native.certify_batch(reqs, results, th)
+        best_c = min(best_c, time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        st = native.check_batch(betas, omegas, ns, mm, certs.values, certs.value_len, th)
+        best_k = min(best_k, time.perf_counter() - t0)
+    certified = int((certs.status == 0).sum())
+    valid = int((st == 0).sum())
+    return {"threads": th, "certified": certified, "checked_valid": valid,
+            "certify_per_s": round(certified / best_c, 1),
+            "check_per_s": round(valid / best_k, 1)}
 
 
 def plain_rate(native, reqs, count, slots):
@@ -167,7 +194,7 @@ def main() -> int:
     ap.add_argument("--served-rate", type=float, default=20000.0)
     ap.add_argument("--served-slots", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--skip", default="", help="comma list of: plain,served,cpu,probe")
+    ap.add_argument("--skip", default="", help="comma list of: plain,served,cpu,probe,cert")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (twice) for counter "
                          "calibration and exit")
@@ -227,6 +254,9 @@ def main() -> int:
         extra["served"] = served_latency(native, args.served_rate, args.served_requests,
                                          args.served_slots, 512, 1 << 30)
         log("served", extra["served"])
+    if "cert" not in skip:
+        extra["certificates"] = certificate_rates(native, reqs, run.results)
+        log("certificates", extra["certificates"])
     cpu = None
     if "cpu" not in skip:
         cpu = cpu_baseline(native, ns, ks, perms)

@@ -265,6 +265,37 @@ int td16_codec_decode_key(const uint8_t *in, int nbytes, int n, uint8_t *omega,
 int td16_codec_decode_value(const uint8_t *in, int nbytes, int32_t *symbols, int cap,
                             int32_t *out_len, char *msg, int msg_cap);
 
+/* Batched certificate work on host threads (threads 0 = every hardware
+ * thread).  Fixed-stride records: key q at keys + q*TD16_CERT_KEY_BYTES,
+ * value q at values + q*TD16_CERT_VALUE_BYTES, with value_len[q] bytes used.
+ *
+ * td16_certify_batch: the emitter's step after a batch of searches
+ * (GPUSortingSearch.java:229-239 positions -> symbols,
+ * CertificateValidator.java:14-50 replay, CertificateCodec.java:22-70 key and
+ * value).  status[q]: 0 certified (key, value written); 1 nothing to certify
+ * (result not OK or no witness; value_len 0); -1..-6 the replay rejected the
+ * witness (codes below).
+ *
+ * td16_check_batch: the checker's lookup of a batch of proof obligations
+ * after the multi-get (checker/.../CertificateStore.java:97-122): each stored
+ * value is decoded (CertificateCodec.java:75-97) and replayed from the
+ * obligation's pair (beta q, omega q at stride TD16_MAX_N, ns[q] symbols)
+ * under max_moves[q].  status[q]: 0 valid; 1 no stored value (value_len 0,
+ * the reference's empty list); -1 length outside 1..max_moves, -2 beta or
+ * omega not a permutation, -3 a move without three distinct symbols, -4 not
+ * an ordered triple, -5 cyclic target not preserved, -6 not sorted, -7
+ * malformed value (length or padding).  The reference throws on the first
+ * invalid certificate; here every obligation gets its own status. */
+#define TD16_CERT_KEY_BYTES 11   /* (5 * TD16_MAX_N + 7) / 8 */
+#define TD16_CERT_VALUE_BYTES 17 /* (15 * TD16_MAX_DEPTH + 7) / 8 */
+int td16_certify_batch(const td16_request *reqs, const td16_result *results, int count,
+                       int threads, uint8_t *keys, uint8_t *values, int32_t *value_len,
+                       int32_t *status, char *msg, int msg_cap);
+int td16_check_batch(const uint8_t *betas, const uint8_t *omegas, const int32_t *ns,
+                     const int32_t *max_moves, int count, const uint8_t *values,
+                     const int32_t *value_len, int threads, int32_t *status,
+                     char *msg, int msg_cap);
+
 /* ---------------------------------------------------------------------- */
 /* Synthetic instances                                                     */
 /* ---------------------------------------------------------------------- */

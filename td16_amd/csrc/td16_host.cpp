@@ -6,6 +6,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "../../include/td16_abi.h"
 #include "td16_internal.h"
@@ -218,51 +220,81 @@ int td16_positions_to_symbols(const uint8_t *beta, int n, const int32_t *positio
   return TD16_OK;
 }
 
-int td16_validate(const uint8_t *beta0, const uint8_t *omega0, int n, const int32_t *sym, int len,
-                  int maxMoves, char *msg, int cap) {
-  // emitter/src/main/java/br/unb/cic/tdp/base/CertificateValidator.java:14-50
-  if (len < 1 || len > maxMoves)
-    return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
-                   "Certificate has %d moves; the required range is 1..%d", len, maxMoves);
-  if (n < 1 || n > td16::kMaxN || !td16::is_perm(beta0, n) || !td16::is_perm(omega0, n))
-    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "beta and omega must be permutations");
-  uint8_t beta[td16::kMaxN], omega[td16::kMaxN], rho[td16::kMaxN], nxt[td16::kMaxN];
+}  // extern "C"
+
+namespace td16 {
+
+// Replay of symbol triples (emitter/src/main/java/br/unb/cic/tdp/base/
+// CertificateValidator.java:14-50).  Returns 0 when valid, else the reason:
+// 1 length outside 1..maxMoves, 2 beta or omega not a permutation, 3 a move
+// without three distinct symbols, 4 not an ordered triple in the current
+// cyclic order, 5 the cyclic target not preserved, 6 omega not sorted at the
+// end.  *bad receives the offending move for 3..5.
+int replay(const uint8_t *beta0, const uint8_t *omega0, int n, const int32_t *sym, int len,
+           int maxMoves, int *bad) {
+  if (len < 1 || len > maxMoves) return 1;
+  if (n < 1 || n > kMaxN || !is_perm(beta0, n) || !is_perm(omega0, n)) return 2;
+  uint8_t beta[kMaxN], omega[kMaxN], rho[kMaxN], nxt[kMaxN];
   std::memcpy(beta, beta0, (size_t)n);
   std::memcpy(omega, omega0, (size_t)n);
   for (int p = 0; p < n; ++p) nxt[beta[p]] = beta[(p + 1) % n];
   for (int x = 0; x < n; ++x) rho[x] = omega[nxt[x]];
   for (int m = 0; m < len; ++m) {
+    *bad = m;
     const int a = sym[3 * m], b = sym[3 * m + 1], c = sym[3 * m + 2];
     if (a == b || b == c || a == c || a < 0 || b < 0 || c < 0 || a >= n || b >= n || c >= n)
-      return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "A move must contain three distinct symbols");
+      return 3;
     int pa = -1, pb = -1, pc = -1;
     for (int p = 0; p < n; ++p) {
       if (beta[p] == a) pa = p;
       if (beta[p] == b) pb = p;
       if (beta[p] == c) pc = p;
     }
-    if (!(pa < pb && pb < pc))
-      return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
-                     "Move is not an ordered triple in the current cyclic order: (%d %d %d)", a, b,
-                     c);
+    if (!(pa < pb && pb < pc)) return 4;
     // omega <- omega o (a b c)^-1
     const uint8_t oa = omega[a], ob = omega[b], oc = omega[c];
     omega[a] = oc;
     omega[b] = oa;
     omega[c] = ob;
-    td16::swap_blocks(beta, pa, pb, pc);
+    swap_blocks(beta, pa, pb, pc);
     for (int p = 0; p < n; ++p) nxt[beta[p]] = beta[(p + 1) % n];
     for (int x = 0; x < n; ++x)
-      if (omega[nxt[x]] != rho[x])
-        return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
-                       "Certificate move does not preserve the cyclic target: (%d %d %d)", a, b,
-                       c);
+      if (omega[nxt[x]] != rho[x]) return 5;
   }
   for (int x = 0; x < n; ++x)
-    if (omega[x] != x)
+    if (omega[x] != x) return 6;
+  return 0;
+}
+
+}  // namespace td16
+
+extern "C" {
+
+int td16_validate(const uint8_t *beta0, const uint8_t *omega0, int n, const int32_t *sym, int len,
+                  int maxMoves, char *msg, int cap) {
+  int bad = 0;
+  const int r = td16::replay(beta0, omega0, n, sym, len, maxMoves, &bad);
+  const int32_t *mv = sym + 3 * bad;
+  switch (r) {
+    case 0: return TD16_OK;
+    case 1:
+      return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
+                     "Certificate has %d moves; the required range is 1..%d", len, maxMoves);
+    case 2: return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "beta and omega must be permutations");
+    case 3:
+      return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "A move must contain three distinct symbols");
+    case 4:
+      return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
+                     "Move is not an ordered triple in the current cyclic order: (%d %d %d)",
+                     mv[0], mv[1], mv[2]);
+    case 5:
+      return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
+                     "Certificate move does not preserve the cyclic target: (%d %d %d)", mv[0],
+                     mv[1], mv[2]);
+    default:
       return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
                      "Certificate does not sort the algebraic permutation");
-  return TD16_OK;
+  }
 }
 
 // emitter/src/main/java/br/unb/cic/tdp/base/CertificateCodec.java:22-118
@@ -348,6 +380,135 @@ int td16_codec_decode_value(const uint8_t *in, int nbytes, int32_t *sym, int sym
   if (moves > symCap) return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "output buffer too small");
   for (int q = 0; q < 3 * moves; ++q) sym[q] = read5(in, 5 * q);
   *outLen = moves;
+  return TD16_OK;
+}
+
+}  // extern "C"
+
+namespace td16 {
+namespace {
+
+// Runs body(lo, hi) over [0, count) on `threads` host threads (0: all).
+template <class Body>
+void parallel_ranges(int count, int threads, Body body) {
+  int t = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+  if (t < 1) t = 1;
+  if (t > 256) t = 256;
+  if (t > count) t = count > 0 ? count : 1;
+  if (t == 1) {
+    body(0, count);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve((size_t)t);
+  for (int q = 0; q < t; ++q) {
+    const int lo = (int)((int64_t)count * q / t), hi = (int)((int64_t)count * (q + 1) / t);
+    pool.emplace_back([=] { body(lo, hi); });
+  }
+  for (auto &th : pool) th.join();
+}
+
+void encode5(const uint8_t *v, int count, uint8_t *out) {
+  std::memset(out, 0, (size_t)((5 * count + 7) / 8));
+  int bit = 0;
+  for (int q = 0; q < count; ++q) bit = write5(out, bit, v[q]);
+}
+
+}  // namespace
+}  // namespace td16
+
+extern "C" {
+
+int td16_certify_batch(const td16_request *reqs, const td16_result *results, int count,
+                       int threads, uint8_t *keys, uint8_t *values, int32_t *value_len,
+                       int32_t *status, char *msg, int cap) {
+  if (count < 0 || (count > 0 && (reqs == nullptr || results == nullptr || keys == nullptr ||
+                                   values == nullptr || value_len == nullptr ||
+                                   status == nullptr)))
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "null buffer or negative count");
+  td16::parallel_ranges(count, threads, [&](int lo, int hi) {
+    for (int q = lo; q < hi; ++q) {
+      const td16_request &r = reqs[q];
+      const td16_result &res = results[q];
+      uint8_t *key = keys + (size_t)q * TD16_CERT_KEY_BYTES;
+      uint8_t *val = values + (size_t)q * TD16_CERT_VALUE_BYTES;
+      std::memset(key, 0, TD16_CERT_KEY_BYTES);
+      value_len[q] = 0;
+      if (res.status != TD16_OK || !res.found || res.depth < 1) {
+        status[q] = 1;  // nothing to certify
+        continue;
+      }
+      const int n = r.n;
+      if (n < 1 || n > td16::kMaxN || !td16::is_perm(r.beta, n) || res.depth > TD16_MAX_DEPTH) {
+        status[q] = -2;
+        continue;
+      }
+      // positions -> symbols on a copy of beta (GPUSortingSearch.java:229-239)
+      int32_t sym[3 * TD16_MAX_DEPTH];
+      uint8_t cur[td16::kMaxN];
+      std::memcpy(cur, r.beta, (size_t)n);
+      bool ok = true;
+      for (int m = 0; m < res.depth && ok; ++m) {
+        const int i = res.moves[m][0], j = res.moves[m][1], k = res.moves[m][2];
+        if (!(0 <= i && i < j && j < k && k < n)) {
+          ok = false;
+          break;
+        }
+        sym[3 * m] = cur[i];
+        sym[3 * m + 1] = cur[j];
+        sym[3 * m + 2] = cur[k];
+        td16::swap_blocks(cur, i, j, k);
+      }
+      int bad = 0;
+      const int rr = ok ? td16::replay(r.beta, r.omega, n, sym, res.depth, r.max_moves, &bad) : 4;
+      if (rr != 0) {
+        status[q] = -rr;
+        continue;
+      }
+      td16::encode5(r.omega, n, key);
+      uint8_t sym8[3 * TD16_MAX_DEPTH];
+      for (int s = 0; s < 3 * res.depth; ++s) sym8[s] = (uint8_t)sym[s];
+      td16::encode5(sym8, 3 * res.depth, val);
+      value_len[q] = (15 * res.depth + 7) / 8;
+      status[q] = 0;
+    }
+  });
+  return TD16_OK;
+}
+
+int td16_check_batch(const uint8_t *betas, const uint8_t *omegas, const int32_t *ns,
+                     const int32_t *max_moves, int count, const uint8_t *values,
+                     const int32_t *value_len, int threads, int32_t *status, char *msg, int cap) {
+  if (count < 0 || (count > 0 && (betas == nullptr || omegas == nullptr || ns == nullptr ||
+                                   max_moves == nullptr || values == nullptr ||
+                                   value_len == nullptr || status == nullptr)))
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "null buffer or negative count");
+  td16::parallel_ranges(count, threads, [&](int lo, int hi) {
+    for (int q = lo; q < hi; ++q) {
+      const int nb = value_len[q];
+      if (nb == 0) {
+        status[q] = 1;  // no stored certificate: the obligation stays open
+        continue;
+      }
+      const uint8_t *in = values + (size_t)q * TD16_CERT_VALUE_BYTES;
+      const int moves = nb > 0 ? (8 * nb) / 15 : 0;
+      if (nb < 0 || nb > TD16_CERT_VALUE_BYTES || moves == 0 || moves > TD16_MAX_DEPTH ||
+          !payload_ok(in, nb, 15L * moves)) {
+        status[q] = -7;  // malformed value (CertificateCodec.decodeValue rejects it)
+        continue;
+      }
+      int32_t sym[3 * TD16_MAX_DEPTH];
+      for (int s = 0; s < 3 * moves; ++s) sym[s] = read5(in, 5 * s);
+      int bad = 0;
+      const int n = ns[q];
+      const int rr = (n < 1 || n > td16::kMaxN)
+                         ? 2
+                         : td16::replay(betas + (size_t)q * TD16_MAX_N,
+                                        omegas + (size_t)q * TD16_MAX_N, n, sym, moves,
+                                        max_moves[q], &bad);
+      status[q] = -rr;
+    }
+  });
   return TD16_OK;
 }
 

@@ -14,6 +14,7 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
+from . import _lib as _lib_mod
 from ._lib import (FIRST, COUNT_ALL, MOVES_ALL, MOVES_REF, BatchOptions, BatchStats,
                    Request, Result, ServeOptions, ServeStats, Td16Error, check, lib, msgbuf)
 
@@ -262,6 +263,71 @@ def codec_decode_value(data: bytes) -> List[Move]:
     check(lib().td16_codec_decode_value(_u8(list(data)), len(data), sym, 32, ctypes.byref(ln),
                                         m, len(m)), m)
     return [(sym[3 * q], sym[3 * q + 1], sym[3 * q + 2]) for q in range(ln.value)]
+
+
+@dataclass
+class Certificates:
+    """Fixed-stride certificate records of a batch (td16_certify_batch)."""
+    keys: np.ndarray       # uint8 [count, CERT_KEY_BYTES]; key q uses (5 n_q + 7) // 8 bytes
+    values: np.ndarray     # uint8 [count, CERT_VALUE_BYTES]
+    value_len: np.ndarray  # int32 [count]; 0 = nothing certified
+    status: np.ndarray     # int32 [count]; 0 certified, 1 nothing to certify, <0 rejected
+
+    def key(self, q: int, n: int) -> bytes:
+        return bytes(self.keys[q, :(5 * n + 7) // 8])
+
+    def value(self, q: int) -> bytes:
+        return bytes(self.values[q, :self.value_len[q]])
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def certify_batch(reqs: Sequence[Request], results: Sequence[Result],
+                  threads: int = 0) -> Certificates:
+    """Positions -> symbols -> replay -> codec records for a batch of search
+    results, on host threads (GPUSortingSearch.java:229-239,
+    CertificateValidator.java:14-50, CertificateCodec.java:22-70)."""
+    n = len(reqs)
+    if len(results) != n:
+        raise ValueError("reqs and results differ in length")
+    rarr = reqs if isinstance(reqs, ctypes.Array) else (Request * max(1, n))(*reqs)
+    oarr = results if isinstance(results, ctypes.Array) else (Result * max(1, n))(*results)
+    keys = np.zeros((max(1, n), _lib_mod.CERT_KEY_BYTES), dtype=np.uint8)
+    vals = np.zeros((max(1, n), _lib_mod.CERT_VALUE_BYTES), dtype=np.uint8)
+    vlen = np.zeros(max(1, n), dtype=np.int32)
+    st = np.zeros(max(1, n), dtype=np.int32)
+    m = msgbuf()
+    check(lib().td16_certify_batch(rarr, oarr, n, int(threads), _ptr(keys, ctypes.c_uint8),
+                                   _ptr(vals, ctypes.c_uint8), _ptr(vlen, ctypes.c_int32),
+                                   _ptr(st, ctypes.c_int32), m, len(m)), m)
+    return Certificates(keys[:n], vals[:n], vlen[:n], st[:n])
+
+
+def check_batch(betas: np.ndarray, omegas: np.ndarray, ns: np.ndarray, max_moves,
+                values: np.ndarray, value_len: np.ndarray, threads: int = 0) -> np.ndarray:
+    """Decodes and replays a batch of stored certificates against their proof
+    obligations (checker/.../CertificateStore.java:97-122).  betas, omegas:
+    uint8 [count, MAX_N]; values: uint8 [count, CERT_VALUE_BYTES]; value_len 0
+    means no stored value.  Returns int32 status per obligation (0 valid, 1 no
+    value, -1..-6 replay rejection, -7 malformed value)."""
+    count = len(ns)
+    b = np.ascontiguousarray(betas, dtype=np.uint8).reshape(-1, _lib_mod.MAX_N)
+    o = np.ascontiguousarray(omegas, dtype=np.uint8).reshape(-1, _lib_mod.MAX_N)
+    v = np.ascontiguousarray(values, dtype=np.uint8).reshape(-1, _lib_mod.CERT_VALUE_BYTES)
+    nn = np.ascontiguousarray(ns, dtype=np.int32)
+    mm = np.ascontiguousarray(np.broadcast_to(np.asarray(max_moves, dtype=np.int32), (count,)))
+    vl = np.ascontiguousarray(value_len, dtype=np.int32)
+    if not (len(b) >= count and len(o) >= count and len(v) >= count and len(vl) >= count):
+        raise ValueError("check_batch: arrays shorter than ns")
+    st = np.zeros(max(1, count), dtype=np.int32)
+    m = msgbuf()
+    check(lib().td16_check_batch(_ptr(b, ctypes.c_uint8), _ptr(o, ctypes.c_uint8),
+                                 _ptr(nn, ctypes.c_int32), _ptr(mm, ctypes.c_int32), count,
+                                 _ptr(v, ctypes.c_uint8), _ptr(vl, ctypes.c_int32), int(threads),
+                                 _ptr(st, ctypes.c_int32), m, len(m)), m)
+    return st[:count]
 
 
 # ---------------------------------------------------------------------------

@@ -111,6 +111,122 @@ def test_codec_matches_oracle(O, native):
         native.codec_value([(0, 1, 32)])
 
 
+# oracle replay codes -> td16_check_batch / td16_certify_batch status
+_ORACLE_TO_STATUS = {0: 0, -1: -1, -6: -2, -2: -3, -3: -4, -4: -5, -5: -6}
+
+
+def _oracle_results(O, native, seed, count):
+    ns, ks, perms = O.gen_products(seed, 5, 12, 1, 6, count)
+    reqs, results, syms = [], [], []
+    for q in range(count):
+        req = native.linear_request(int(perms[q]), int(ns[q]), int(ks[q]))
+        beta, omega = list(req.beta[:req.n]), list(req.omega[:req.n])
+        # every third request gets a tighter bound, so some have no witness
+        mm = int(ks[q]) - (1 if q % 3 == 0 and ks[q] > 1 else 0)
+        req.max_moves = mm
+        r = O.search(beta, omega, req.initial_even, req.min_rate, mm, True)
+        res = _lib.Result()
+        res.found = r.found
+        w = r.witness() if r.found else []
+        res.depth = len(w)
+        for d, mv in enumerate(w):
+            res.moves[d][0], res.moves[d][1], res.moves[d][2] = mv
+        reqs.append(req)
+        results.append(res)
+        syms.append(O.positions_to_symbols(beta, w) if w else [])
+    return reqs, results, syms
+
+
+def test_certify_batch_matches_oracle_codec_and_replay(O, native):
+    reqs, results, syms = _oracle_results(O, native, 31, 240)
+    # a device error and an invalid position triple are not certified
+    results[5].status = _lib.DEVICE_ERROR
+    bad = next(q for q in range(6, 240) if results[q].found and results[q].depth >= 1)
+    results[bad].moves[0][0], results[bad].moves[0][1] = results[bad].moves[0][1], \
+        results[bad].moves[0][0]
+    for threads in (1, 3, 0):
+        certs = native.certify_batch(reqs, results, threads)
+        for q, req in enumerate(reqs):
+            n = req.n
+            beta, omega = list(req.beta[:n]), list(req.omega[:n])
+            if q == 5 or not results[q].found:
+                assert certs.status[q] == 1 and certs.value_len[q] == 0
+                continue
+            if q == bad:
+                assert certs.status[q] == -4 and certs.value_len[q] == 0
+                continue
+            assert certs.status[q] == 0
+            assert O.validate(beta, omega, syms[q], req.max_moves) == 0
+            assert certs.key(q, n) == O.codec_key(omega)
+            assert certs.value(q) == O.codec_value(syms[q])
+            assert O.codec_decode_value(certs.value(q)) == syms[q]
+    assert (certs.status == 1).sum() >= 10 and (certs.status == 0).sum() >= 100
+    empty = native.certify_batch([], [])
+    assert len(empty.status) == 0
+
+
+def test_check_batch_matches_oracle_replay(O, native):
+    reqs, results, syms = _oracle_results(O, native, 32, 300)
+    certs = native.certify_batch(reqs, results)
+    count = len(reqs)
+    betas = np.zeros((count, _lib.MAX_N), np.uint8)
+    omegas = np.zeros((count, _lib.MAX_N), np.uint8)
+    ns = np.array([r.n for r in reqs], np.int32)
+    mm = np.array([r.max_moves for r in reqs], np.int32)
+    for q, r in enumerate(reqs):
+        betas[q, :r.n] = r.beta[:r.n]
+        omegas[q, :r.n] = r.omega[:r.n]
+    values = certs.values.copy()
+    vlen = certs.value_len.copy()
+    rng = random.Random(4)
+    expect_malformed = set()
+    stored = [q for q in range(count) if vlen[q] > 0]
+    for q in stored:
+        kind = q % 7
+        moves = O.codec_decode_value(bytes(values[q, :vlen[q]]))
+        if kind == 1:                      # a permuted triple
+            a, b, c = moves[0]
+            moves[0] = (a, c, b)
+        elif kind == 2 and len(moves) > 1:  # truncated certificate
+            moves = moves[:-1]
+        elif kind == 3:                    # a foreign symbol
+            moves[-1] = (moves[-1][0], moves[-1][1], rng.randint(ns[q], 31))
+        elif kind == 4:                    # the bound tightened below the length
+            mm[q] = len(moves) - 1
+        elif kind == 5:                    # nonzero padding bit
+            nb = int(vlen[q])
+            pad = 8 * nb - 15 * len(moves)
+            if pad:
+                values[q, nb - 1] |= 1
+                expect_malformed.add(q)
+            continue
+        elif kind == 6:                    # length that is no whole number of moves
+            vlen[q] = 1
+            expect_malformed.add(q)
+            continue
+        v = O.codec_value(moves)
+        values[q, :] = 0
+        values[q, :len(v)] = list(v)
+        vlen[q] = len(v)
+    st = native.check_batch(betas, omegas, ns, mm, values, vlen, threads=4)
+    seen = set()
+    for q in range(count):
+        if vlen[q] == 0:
+            assert st[q] == 1
+        elif q in expect_malformed:
+            assert st[q] == -7
+            with pytest.raises(ValueError):
+                O.codec_decode_value(bytes(values[q, :vlen[q]]))
+        else:
+            n = int(ns[q])
+            moves = O.codec_decode_value(bytes(values[q, :vlen[q]]))
+            code = O.validate(list(betas[q, :n]), list(omegas[q, :n]), moves, int(mm[q]))
+            assert st[q] == _ORACLE_TO_STATUS[code], (q, code, st[q])
+        seen.add(int(st[q]))
+    assert {0, 1, -1, -3, -4, -7} <= seen
+    assert len(native.check_batch(betas[:0], omegas[:0], ns[:0], 9, values[:0], vlen[:0])) == 0
+
+
 def test_search_argument_errors_without_gpu(native):
     # jni_sorter.cu:61-67, sorting_search.cu:820-825: argument errors surface
     # before any device work

@@ -6,14 +6,16 @@ in 12..16, each the product of k ~ U{4..9} uniform block transpositions
 applied to the identity (SplitMix64 seed 1), task: any sorting sequence of
 length <= k under the reference's move filter, first witness wins.  One step
 is one persistent launch over all of them (hipGraph-captured), inputs already
-resident in HBM; the timed region is the control reset, the launch and the
-copy of the results back to the host.
+resident in HBM; the timed region is the control reset and the launch, whose
+results land in coherent pinned host memory as requests finish.
 
 The JSON line reports verified witnesses/s (every witness is replayed by the
-product's certificate checker outside the timed region), nodes/s, the plain
-one-instance-per-launch kernel on the same instances, served-loop p50/p99
-latency under Poisson arrivals, the integer-op roofline of the search kernel
-and the CPU restatement timed on this host.
+product's batched certificate path outside the timed region), nodes/s, the
+plain one-instance-per-launch kernel on the same instances, served-loop
+p50/p99 latency under Poisson arrivals, the integer-op roofline of the search
+kernel, the CPU restatement timed on this host, the host rates of the
+certificate path (certify, check) and an emitter-shaped stream of 1,000,000
+n = 16 instances searched, certified and checked end to end.
 """
 from __future__ import annotations
 
@@ -71,16 +73,7 @@ def certificate_rates(native, reqs, results, repeats=5):
     """Host throughput of the certificate path over one batch's results:
     certify (the emitter's replay + encode) and check (the checker's decode +
     replay of stored values), best of `repeats`, on host_threads() threads."""
-    count = len(results)
-    betas = np.zeros((count, 17), np.uint8)
-    omegas = np.zeros((count, 17), np.uint8)
-    ns = np.zeros(count, np.int32)
-    mm = np.zeros(count, np.int32)
-    for q in range(count):
-        n = reqs[q].n
-        ns[q], mm[q] = n, reqs[q].max_moves
-        betas[q, :n] = reqs[q].beta[:n]
-        omegas[q, :n] = reqs[q].omega[:n]
+    betas, omegas, ns, mm = native.obligations(reqs)
     th = host_threads()
     best_c = best_k = float("inf")
     for _ in range(repeats):
@@ -120,7 +113,7 @@ def plain_rate(native, reqs, count, slots):
         dt = time.perf_counter() - t0
     finally:
         native.shutdown()
-    found, verified = verify(native, reqs, out)
+    found, verified = verify(native, reqs[:count], out)
     return verified / dt, dt, found, verified
 
 
@@ -148,6 +141,44 @@ def cpu_baseline(native, ns, ks, perms, seconds_target=15.0):
         "sample": "first %d instances of the same mix, C restatement of the search in the "
                   "kernel's default (greedy-first) order (walk-based delta) + certificate "
                   "replay, %d threads, %.1f s; %.3g nodes/s" % (count, cores, dt, nodes / dt),
+    }
+
+
+def emitter_stream(native, count):
+    """The emitter-shaped stream without a JVM or RocksDB: `count` n = 16
+    instances, depth bound 9 (9 in 10 products of 9 block transpositions, 1
+    in 10 reverse-like), searched in one batched launch (after one untimed
+    launch of the same batch), every witness certified (replay + codec
+    records) and every stored record checked again by the batched checker."""
+    ns, ks, perms = native.gen_emitter_stream(5, count)
+    reqs = build_requests(native, ns, ks, perms)
+    native.batch(reqs, use_graph=False)
+    run = native.batch(reqs, use_graph=False)
+    st = run.stats
+    th = host_threads()
+    t0 = time.perf_counter()
+    certs = native.certify_batch(reqs, run.results, th)
+    certify_s = time.perf_counter() - t0
+    betas, omegas, nn, mm = native.obligations(reqs)
+    t0 = time.perf_counter()
+    chk = native.check_batch(betas, omegas, nn, mm, certs.values, certs.value_len, th)
+    check_s = time.perf_counter() - t0
+    found = sum(1 for q in range(count) if run.results[q].status == 0 and run.results[q].found)
+    certified = int((certs.status == 0).sum())
+    return {
+        "instances": count,
+        "kernel_ms": round(st.kernel_ms, 3),
+        "wall_ms": round(st.wall_ms, 3),
+        "expanded_nodes": st.expanded,
+        "found": found,
+        "certified": certified,
+        "checked_valid": int((chk == 0).sum()),
+        "certify_ms": round(certify_s * 1e3, 3),
+        "check_ms": round(check_s * 1e3, 3),
+        "host_threads": th,
+        "witnesses_per_s_search": round(certified / (st.wall_ms * 1e-3), 1),
+        "witnesses_per_s_search_plus_certify": round(
+            certified / (st.wall_ms * 1e-3 + certify_s), 1),
     }
 
 
@@ -194,7 +225,9 @@ def main() -> int:
     ap.add_argument("--served-rate", type=float, default=20000.0)
     ap.add_argument("--served-slots", type=int, default=64)
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--skip", default="", help="comma list of: plain,served,cpu,probe,cert")
+    ap.add_argument("--emitter-instances", type=int, default=1000000)
+    ap.add_argument("--skip", default="",
+                    help="comma list of: plain,served,cpu,probe,cert,emitter")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (twice) for counter "
                          "calibration and exit")
@@ -257,6 +290,9 @@ def main() -> int:
     if "cert" not in skip:
         extra["certificates"] = certificate_rates(native, reqs, run.results)
         log("certificates", extra["certificates"])
+    if "emitter" not in skip and args.emitter_instances > 0:
+        extra["emitter_stream"] = emitter_stream(native, args.emitter_instances)
+        log("emitter stream", extra["emitter_stream"])
     cpu = None
     if "cpu" not in skip:
         cpu = cpu_baseline(native, ns, ks, perms)

@@ -657,8 +657,8 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
     if (st != TD16_OK) return st;
   }
   Buffers &b = gBatch.buf;
-  // 0 = the measured best for the batched mix: 12 waves per CU (3 per SIMD);
-  // more resident waves only add idle polling once the feed drains
+  // 0 = the measured best for the batched mix: 16 waves per CU (4 per SIMD,
+  // the most the kernel's 128-VGPR budget allows)
   const int wantWaves = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
   const int perCU = std::max(1, std::min(dev.blocksPerCU, wantWaves / kWavesPerBlock));
   const int blocks = dev.cus * perCU;

@@ -305,6 +305,22 @@ def certify_batch(reqs: Sequence[Request], results: Sequence[Result],
     return Certificates(keys[:n], vals[:n], vlen[:n], st[:n])
 
 
+def obligations(reqs: ctypes.Array):
+    """(betas, omegas, ns, max_moves) of a request array as numpy arrays, the
+    per-obligation inputs of check_batch (a byte view, no per-request loop)."""
+    count = len(reqs)
+    raw = np.frombuffer(memoryview(reqs).cast("B"), dtype=np.uint8).reshape(
+        count, ctypes.sizeof(Request))
+
+    def i32(field):
+        off = getattr(Request, field).offset
+        return raw[:, off:off + 4].copy().view(np.int32).reshape(count)
+
+    b0, o0 = Request.beta.offset, Request.omega.offset
+    mx = _lib_mod.MAX_N
+    return (raw[:, b0:b0 + mx].copy(), raw[:, o0:o0 + mx].copy(), i32("n"), i32("max_moves"))
+
+
 def check_batch(betas: np.ndarray, omegas: np.ndarray, ns: np.ndarray, max_moves,
                 values: np.ndarray, value_len: np.ndarray, threads: int = 0) -> np.ndarray:
     """Decodes and replays a batch of stored certificates against their proof
@@ -352,6 +368,43 @@ def gen_products(seed: int, nmin: int, nmax: int, kmin: int, kmax: int, count: i
                                  perms.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     if st != 0:
         raise Td16Error(st, "invalid generator arguments")
+    return ns, ks, perms
+
+
+def gen_reverse_like(seed: int, n: int, count: int, rmax: int = 2) -> np.ndarray:
+    """Nibble-packed permutations near the reverse permutation n-1 .. 0: the
+    reverse followed by r ~ U{0..rmax} block transpositions (positions
+    i < j < k uniform over 0..n, as in gen_products).  The reverse of 16 has
+    the maximum distance 9 at that size; its neighbours are the hard cases of
+    an emitter-shaped stream.  Python's Mersenne Twister seeded with `seed`."""
+    import random
+    rng = random.Random(seed)
+    out = np.zeros(count, dtype=np.uint64)
+    for c in range(count):
+        p = list(range(n - 1, -1, -1))
+        for _ in range(rng.randint(0, rmax)):
+            i, j, k = sorted(rng.sample(range(n + 1), 3))
+            p = p[:i] + p[j:k] + p[i:j] + p[k:]
+        v = 0xFFFFFFFFFFFFFFFF
+        for q in range(n):
+            v = (v & ~(0xF << (4 * q))) | (p[q] << (4 * q))
+        out[c] = v
+    return out
+
+
+def gen_emitter_stream(seed: int, count: int, reverse_every: int = 10):
+    """An emitter-shaped stream of n = 16 instances, depth bound 9: every
+    `reverse_every`-th instance is reverse-like (gen_reverse_like), the rest
+    are products of 9 random block transpositions (gen_products).  Returns
+    (ns, ks, perms) like gen_products."""
+    hard = np.arange(count) % reverse_every == reverse_every - 1
+    nh = int(hard.sum())
+    ns = np.full(count, 16, dtype=np.uint8)
+    ks = np.full(count, 9, dtype=np.uint8)
+    perms = np.zeros(count, dtype=np.uint64)
+    _, _, easy = gen_products(seed, 16, 16, 9, 9, count - nh)
+    perms[~hard] = easy
+    perms[hard] = gen_reverse_like(seed + 1, 16, nh)
     return ns, ks, perms
 
 

@@ -158,3 +158,37 @@ def test_batch_sizes_around_feed_shards(O, native, count):
             assert r.found == 1 and r.depth == 0
         else:
             check_against_oracle(O, native, req, r)
+
+
+def test_emitter_stream_certified_and_checked(O, native):
+    """An emitter-shaped stream (n = 16, depth bound 9, one instance in ten
+    reverse-like): the first 1,000 against the oracle (greedy order), and every
+    witness of all 20,000 certified (replay + codec records) and checked again
+    from its stored record by the batched checker."""
+    count = 20000
+    ns, ks, perms = native.gen_emitter_stream(5, count)
+    reqs = [native.linear_request(int(perms[q]), int(ns[q]), int(ks[q])) for q in range(count)]
+    run = native.batch(reqs)
+    sample = 1000
+    ref = O.search_batch(ns[:sample], perms[:sample], ks[:sample], O.MOVES_REF, O.FIRST,
+                         threads=8, greedy=True)
+    for q in range(sample):
+        r = run.results[q]
+        assert r.status == 0 and r.found == ref[q].found
+        if not r.split:
+            assert r.witness() == ref[q].witness() and r.expanded == ref[q].expanded
+    found = np.array([run.results[q].status == 0 and run.results[q].found == 1
+                      for q in range(count)])
+    certs = native.certify_batch(reqs, run.results)
+    assert np.array_equal(certs.status == 0, found)
+    assert np.all(certs.status[~found] == 1)
+    betas, omegas, nn, mm = native.obligations((type(reqs[0]) * count)(*reqs))
+    st = native.check_batch(betas, omegas, nn, mm, certs.values, certs.value_len)
+    assert np.array_equal(st == 0, found) and np.all(st[~found] == 1)
+    # a record is the codec of the witness's symbols under the request's omega key
+    for q in range(0, count, 997):
+        if found[q]:
+            beta, omega = pair_of(reqs[q])
+            sym = native.positions_to_symbols(beta, run.results[q].witness())
+            assert certs.value(q) == O.codec_value(sym)
+            assert certs.key(q, reqs[q].n) == O.codec_key(omega)

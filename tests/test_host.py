@@ -227,6 +227,35 @@ def test_check_batch_matches_oracle_replay(O, native):
     assert len(native.check_batch(betas[:0], omegas[:0], ns[:0], 9, values[:0], vlen[:0])) == 0
 
 
+def test_emitter_stream_generator_and_obligation_view(O, native):
+    def unpack(v, n):
+        return [(int(v) >> (4 * q)) & 0xF for q in range(n)]
+
+    rev = native.gen_reverse_like(3, 16, 300, rmax=0)
+    assert all(unpack(v, 16) == list(range(15, -1, -1)) for v in rev)
+    short = native.gen_reverse_like(3, 12, 5, rmax=0)
+    assert all(int(v) >> 48 == 0xFFFF for v in short)           # unused nibbles stay 0xF
+    ns, ks, perms = native.gen_emitter_stream(5, 2000)
+    assert set(ns) == {16} and set(ks) == {9}
+    for q in range(2000):
+        p = unpack(perms[q], 16)
+        assert sorted(p) == list(range(16))
+        if q % 10 == 9:
+            # reverse-like: at most two block transpositions from the reverse,
+            # so at least 16 - 2*3 adjacent pairs stay descending
+            assert sum(p[i] == p[i + 1] + 1 for i in range(15)) >= 15 - 6
+    _, _, easy = native.gen_products(5, 16, 16, 9, 9, 1800)
+    assert np.array_equal(perms[np.arange(2000) % 10 != 9], easy)
+    reqs = [native.linear_request(int(perms[q]), 16, 9) for q in range(50)]
+    arr = (_lib.Request * 50)(*reqs)
+    betas, omegas, nn, mm = native.obligations(arr)
+    for q, r in enumerate(reqs):
+        assert nn[q] == r.n == 17 and mm[q] == 9
+        assert list(betas[q, :17]) == list(r.beta) and list(omegas[q, :17]) == list(r.omega)
+        beta, omega = O.linear_to_pair(perms[q], 16)
+        assert list(betas[q, :17]) == beta and list(omegas[q, :17]) == omega
+
+
 def test_search_argument_errors_without_gpu(native):
     # jni_sorter.cu:61-67, sorting_search.cu:820-825: argument errors surface
     # before any device work

@@ -319,6 +319,36 @@ int td16_gen_exponential(uint64_t seed, int count, double *gaps);
 int td16_linear_request(uint64_t perm, int n, int k, int moveset, int mode, int iddfs,
                         td16_request *req);
 
+/* ---------------------------------------------------------------------- */
+/* Producer side: cycle-configuration representatives                      */
+/* ---------------------------------------------------------------------- */
+
+#define TD16_BRACELET_PAIRS 0
+#define TD16_BRACELET_WORDS 1
+
+/* One representative per realizable extended-toric class of an annotated
+ * partition -- omega's cycle lengths parts[0..m-1], oriented[q] != 0 for an
+ * oriented cycle -- the set DirectTwistedBraceletGenerator emits
+ * (DirectTwistedBraceletGenerator.java:78-104 generateRepresentatives and
+ * countRepresentatives; validation :234-255).
+ *   mode TD16_BRACELET_PAIRS: a row is omega's one-line notation (n bytes,
+ *     n = sum of parts) of the pair (omega, beta = (0 1 .. n-1)), whose
+ *     target omega o beta is one n-cycle (the reference's CyclicTargetPair
+ *     from decodeWord, :1475-1505);
+ *   mode TD16_BRACELET_WORDS: a row is a raw orbit-minimal word before the
+ *     target filter (the reference's generateWords test hook, :225-232):
+ *     symbols number the cycles longest first (unoriented before oriented of
+ *     one length), one symbol per unoriented cycle and one per rank of an
+ *     oriented cycle.
+ * out: cap_rows rows of n bytes, or NULL to count only; *total receives the
+ * number of rows.  When it exceeds cap_rows, the first cap_rows rows are
+ * written and TD16_INVALID_ARGUMENT is returned with the count in the
+ * message.  Rows come in increasing word order for any `threads` (0 = all
+ * hardware threads).  n <= 32; a searchable pair needs n <= TD16_MAX_N. */
+int td16_bracelets(const int32_t *parts, const uint8_t *oriented, int m, int mode,
+                   int threads, uint8_t *out, int64_t cap_rows, int64_t *total,
+                   char *msg, int msg_cap);
+
 /* Sustained int32 VALU rate of the device, in 10^12 lane-ops per second:
  * a grid of resident wavefronts runs independent dependent chains of
  * v_add_u32 / v_xor_b32 for `iterations` rounds.  Measurement aid for the

@@ -16,6 +16,7 @@ MAX_N = 17
 MAX_DEPTH = 9
 CERT_KEY_BYTES = 11
 CERT_VALUE_BYTES = 17
+BRACELET_PAIRS, BRACELET_WORDS = 0, 1
 
 OK = 0
 INVALID_ARGUMENT = 1
@@ -166,6 +167,8 @@ SIGNATURES = {
     "td16_certify_batch": (_i, [ctypes.POINTER(Request), ctypes.POINTER(Result), _i, _i, _u8p,
                                 _u8p, _i32p, _i32p, _c, _i]),
     "td16_check_batch": (_i, [_u8p, _u8p, _i32p, _i32p, _i, _u8p, _i32p, _i, _i32p, _c, _i]),
+    "td16_bracelets": (_i, [_i32p, _u8p, _i, _i, _i, _u8p, ctypes.c_int64,
+                            ctypes.POINTER(ctypes.c_int64), _c, _i]),
     "td16_gen_uniform": (_i, [ctypes.c_uint64, _i, _i, _u64p]),
     "td16_gen_products": (_i, [ctypes.c_uint64, _i, _i, _i, _i, _i, _u8p, _u8p, _u64p]),
     "td16_gen_exponential": (_i, [ctypes.c_uint64, _i, ctypes.POINTER(ctypes.c_double)]),

@@ -189,6 +189,7 @@ int launch_queue_init(uint64_t *seq, uint64_t capPerShard, void *stream);
 int search_kernel_occupancy(int *blocksPerCU);
 
 // Host helpers (td16_host.cpp).
+int set_msg(char *msg, int cap, int status, const char *fmt, ...);
 void build_lex_table(uint16_t *out);   // kLexEntries entries
 int rate_tables(int N, int initialEven, float minRate, int maxDepth, int fullSorting,
                 uint32_t sat[10], uint32_t reach[10]);

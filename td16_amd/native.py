@@ -347,6 +347,100 @@ def check_batch(betas: np.ndarray, omegas: np.ndarray, ns: np.ndarray, max_moves
 
 
 # ---------------------------------------------------------------------------
+# producer side: cycle-configuration representatives
+# ---------------------------------------------------------------------------
+
+def _bracelet_args(parts, oriented):
+    m = len(parts)
+    if oriented is None:
+        oriented = [False] * m
+    if len(oriented) != m:
+        raise Td16Error(_lib_mod.INVALID_ARGUMENT,
+                        "Partition and orientation arrays must have the same length")
+    return (m, (ctypes.c_int32 * max(1, m))(*[int(p) for p in parts]),
+            (ctypes.c_uint8 * max(1, m))(*[1 if o else 0 for o in oriented]))
+
+
+def bracelet_count(parts: Sequence[int], oriented: Optional[Sequence[bool]] = None,
+                   threads: int = 0, words: bool = False) -> int:
+    """Number of realizable representatives (countRepresentatives,
+    DirectTwistedBraceletGenerator.java:94-101), or of raw words."""
+    m, P, O = _bracelet_args(parts, oriented)
+    total = ctypes.c_int64(0)
+    msg = msgbuf()
+    mode = _lib_mod.BRACELET_WORDS if words else _lib_mod.BRACELET_PAIRS
+    check(lib().td16_bracelets(P, O, m, mode, int(threads), None, 0, ctypes.byref(total),
+                               msg, len(msg)), msg)
+    return total.value
+
+
+def bracelets(parts: Sequence[int], oriented: Optional[Sequence[bool]] = None,
+              threads: int = 0, words: bool = False) -> np.ndarray:
+    """Representatives of an annotated partition as a uint8 array [rows, n]:
+    omega's one-line notation of each pair (omega, beta = (0 1 .. n-1)), or
+    with words=True the raw orbit-minimal words (generateWords)."""
+    m, P, O = _bracelet_args(parts, oriented)
+    n = int(sum(int(p) for p in parts))
+    total = ctypes.c_int64(0)
+    mode = _lib_mod.BRACELET_WORDS if words else _lib_mod.BRACELET_PAIRS
+    cap = 1 << 12
+    while True:
+        out = np.zeros((cap, max(1, n)), dtype=np.uint8)
+        msg = msgbuf()
+        st = lib().td16_bracelets(P, O, m, mode, int(threads), _ptr(out, ctypes.c_uint8), cap,
+                                  ctypes.byref(total), msg, len(msg))
+        if st == _lib_mod.OK:
+            return out[:total.value]
+        if total.value > cap:
+            cap = total.value
+            continue
+        check(st, msg)
+
+
+def bracelet_requests(omegas: np.ndarray, target: int, max_moves: int) -> ctypes.Array:
+    """Requests of the emitter's unoriented stage for representative pairs
+    (UnorientedCycles.java:33-58 via WitnessSearch.java:23-34): beta = the
+    identity cycle, omega, initial_even = omega's odd-length cycles,
+    min_rate = 2 / (max_moves / (float) target), full sorting, REF, FIRST."""
+    rows, n = omegas.shape
+    arr = (Request * max(1, rows))()
+    raw = np.frombuffer(memoryview(arr).cast("B"), dtype=np.uint8).reshape(
+        max(1, rows), ctypes.sizeof(Request))[:rows]
+    b0, o0 = Request.beta.offset, Request.omega.offset
+    raw[:, b0:b0 + n] = np.arange(n, dtype=np.uint8)
+    raw[:, o0:o0 + n] = omegas
+    # odd-length cycles of each omega, by walking all rows at once
+    seen = np.zeros((rows, n), dtype=bool)
+    odd = np.zeros(rows, dtype=np.int32)
+    idx = np.arange(rows)
+    for start in range(n):
+        todo = ~seen[:, start]
+        x = np.full(rows, start)
+        length = np.zeros(rows, dtype=np.int32)
+        while True:
+            active = todo & ~seen[idx, x]
+            if not active.any():
+                break
+            seen[idx[active], x[active]] = True
+            length[active] += 1
+            x = np.where(active, omegas[idx, x].astype(np.int64), x)
+        odd += (todo & (length % 2 == 1)).astype(np.int32)
+    rate = np.float32(2) / (np.float32(max_moves) / np.float32(target))
+
+    def put(field, values, dtype):
+        off = getattr(Request, field).offset
+        raw[:, off:off + 4] = np.ascontiguousarray(
+            np.broadcast_to(np.asarray(values, dtype=dtype), (rows,))).view(np.uint8).reshape(rows, 4)
+
+    put("n", n, np.int32)
+    put("initial_even", odd, np.int32)
+    put("min_rate", rate, np.float32)
+    put("max_moves", max_moves, np.int32)
+    put("full_sorting", 1, np.int32)
+    return arr
+
+
+# ---------------------------------------------------------------------------
 # synthetic instances
 # ---------------------------------------------------------------------------
 

@@ -14,8 +14,9 @@ product's batched certificate path outside the timed region), nodes/s, the
 plain one-instance-per-launch kernel on the same instances, served-loop
 p50/p99 latency under Poisson arrivals, the integer-op roofline of the search
 kernel, the CPU restatement timed on this host, the host rates of the
-certificate path (certify, check) and an emitter-shaped stream of 1,000,000
-n = 16 instances searched, certified and checked end to end.
+certificate path (certify, check), an emitter-shaped stream of 1,000,000
+n = 16 instances searched, certified and checked end to end, and the
+emitter's unoriented stage (3,858,884 generated n = 17 classes) the same way.
 """
 from __future__ import annotations
 
@@ -163,7 +164,8 @@ def emitter_stream(native, count):
     t0 = time.perf_counter()
     chk = native.check_batch(betas, omegas, nn, mm, certs.values, certs.value_len, th)
     check_s = time.perf_counter() - t0
-    found = sum(1 for q in range(count) if run.results[q].status == 0 and run.results[q].found)
+    f = native.result_fields(run.results)
+    found = int(((f["status"] == 0) & (f["found"] == 1)).sum())
     certified = int((certs.status == 0).sum())
     return {
         "instances": count,
@@ -179,6 +181,47 @@ def emitter_stream(native, count):
         "witnesses_per_s_search": round(certified / (st.wall_ms * 1e-3), 1),
         "witnesses_per_s_search_plus_certify": round(
             certified / (st.wall_ms * 1e-3 + certify_s), 1),
+    }
+
+
+def unoriented_stage(native):
+    """The emitter's unoriented stage end to end: generate every realizable
+    class of the 21 partitions of 17 (3,858,884 pairs), search them all in
+    one batched launch (after one untimed launch of the same batch), certify
+    every witness and check every record again.  emitter.log reports every
+    class solved (residual=0); `found` must equal `classes`."""
+    from td16_amd import unoriented
+    th = host_threads()
+    t0 = time.perf_counter()
+    sb = unoriented.stage_requests(threads=th)
+    gen_s = time.perf_counter() - t0
+    count = len(sb.requests)
+    native.batch(sb.requests, use_graph=False)
+    run = native.batch(sb.requests, use_graph=False)
+    st = run.stats
+    f = native.result_fields(run.results)
+    t0 = time.perf_counter()
+    certs = native.certify_batch(sb.requests, run.results, th)
+    certify_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    chk = native.check_batch(*native.obligations(sb.requests), certs.values, certs.value_len, th)
+    check_s = time.perf_counter() - t0
+    certified = int((certs.status == 0).sum())
+    return {
+        "classes": count,
+        "partitions": len(sb.cases),
+        "found": int(((f["status"] == 0) & (f["found"] == 1)).sum()),
+        "certified": certified,
+        "checked_valid": int((chk == 0).sum()),
+        "generate_ms": round(gen_s * 1e3, 1),
+        "kernel_ms": round(st.kernel_ms, 3),
+        "wall_ms": round(st.wall_ms, 3),
+        "expanded_nodes": st.expanded,
+        "certify_ms": round(certify_s * 1e3, 1),
+        "check_ms": round(check_s * 1e3, 1),
+        "host_threads": th,
+        "classes_per_s_end_to_end": round(
+            certified / (gen_s + st.wall_ms * 1e-3 + certify_s), 1),
     }
 
 
@@ -227,7 +270,7 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--emitter-instances", type=int, default=1000000)
     ap.add_argument("--skip", default="",
-                    help="comma list of: plain,served,cpu,probe,cert,emitter")
+                    help="comma list of: plain,served,cpu,probe,cert,emitter,stage")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (twice) for counter "
                          "calibration and exit")
@@ -293,6 +336,9 @@ def main() -> int:
     if "emitter" not in skip and args.emitter_instances > 0:
         extra["emitter_stream"] = emitter_stream(native, args.emitter_instances)
         log("emitter stream", extra["emitter_stream"])
+    if "stage" not in skip:
+        extra["unoriented_stage"] = unoriented_stage(native)
+        log("unoriented stage", extra["unoriented_stage"])
     cpu = None
     if "cpu" not in skip:
         cpu = cpu_baseline(native, ns, ks, perms)

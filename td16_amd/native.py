@@ -305,6 +305,33 @@ def certify_batch(reqs: Sequence[Request], results: Sequence[Result],
     return Certificates(keys[:n], vals[:n], vlen[:n], st[:n])
 
 
+def result_fields(results: ctypes.Array) -> dict:
+    """status, found, depth, split (int32) and expanded (uint64) of a result
+    array as numpy arrays (a byte view, no per-result loop)."""
+    count = len(results)
+    raw = np.frombuffer(memoryview(results).cast("B"), dtype=np.uint8).reshape(
+        count, ctypes.sizeof(Result))
+    out = {}
+    for field, dt in (("status", np.int32), ("found", np.int32), ("depth", np.int32),
+                      ("split", np.int32), ("expanded", np.uint64)):
+        off = getattr(Result, field).offset
+        w = np.dtype(dt).itemsize
+        out[field] = raw[:, off:off + w].copy().view(dt).reshape(count)
+    return out
+
+
+def concat_requests(chunks: Sequence[ctypes.Array]) -> ctypes.Array:
+    """One request array holding the given arrays back to back."""
+    total = sum(len(c) for c in chunks)
+    arr = (Request * max(1, total))()
+    at = 0
+    size = ctypes.sizeof(Request)
+    for c in chunks:
+        ctypes.memmove(ctypes.addressof(arr) + at * size, ctypes.addressof(c), len(c) * size)
+        at += len(c)
+    return arr if total else (Request * 0)()
+
+
 def obligations(reqs: ctypes.Array):
     """(betas, omegas, ns, max_moves) of a request array as numpy arrays, the
     per-obligation inputs of check_batch (a byte view, no per-request loop)."""
@@ -397,34 +424,41 @@ def bracelets(parts: Sequence[int], oriented: Optional[Sequence[bool]] = None,
         check(st, msg)
 
 
-def bracelet_requests(omegas: np.ndarray, target: int, max_moves: int) -> ctypes.Array:
+def bracelet_requests(omegas: np.ndarray, target: int, max_moves: int,
+                      parts: Optional[Sequence[int]] = None) -> ctypes.Array:
     """Requests of the emitter's unoriented stage for representative pairs
     (UnorientedCycles.java:33-58 via WitnessSearch.java:23-34): beta = the
-    identity cycle, omega, initial_even = omega's odd-length cycles,
-    min_rate = 2 / (max_moves / (float) target), full sorting, REF, FIRST."""
+    identity cycle, omega, initial_even = omega's odd-length cycles (the odd
+    parts when the rows' partition `parts` is given), min_rate =
+    2 / (max_moves / (float) target), full sorting, REF, FIRST."""
     rows, n = omegas.shape
-    arr = (Request * max(1, rows))()
+    if rows == 0:
+        return (Request * 0)()
+    arr = (Request * rows)()
     raw = np.frombuffer(memoryview(arr).cast("B"), dtype=np.uint8).reshape(
-        max(1, rows), ctypes.sizeof(Request))[:rows]
+        rows, ctypes.sizeof(Request))
     b0, o0 = Request.beta.offset, Request.omega.offset
     raw[:, b0:b0 + n] = np.arange(n, dtype=np.uint8)
     raw[:, o0:o0 + n] = omegas
-    # odd-length cycles of each omega, by walking all rows at once
-    seen = np.zeros((rows, n), dtype=bool)
-    odd = np.zeros(rows, dtype=np.int32)
-    idx = np.arange(rows)
-    for start in range(n):
-        todo = ~seen[:, start]
-        x = np.full(rows, start)
-        length = np.zeros(rows, dtype=np.int32)
-        while True:
-            active = todo & ~seen[idx, x]
-            if not active.any():
-                break
-            seen[idx[active], x[active]] = True
-            length[active] += 1
-            x = np.where(active, omegas[idx, x].astype(np.int64), x)
-        odd += (todo & (length % 2 == 1)).astype(np.int32)
+    if parts is not None:
+        odd = sum(1 for p in parts if p % 2 == 1)
+    else:
+        # odd-length cycles of each omega, by walking all rows at once
+        seen = np.zeros((rows, n), dtype=bool)
+        odd = np.zeros(rows, dtype=np.int32)
+        idx = np.arange(rows)
+        for start in range(n):
+            todo = ~seen[:, start]
+            x = np.full(rows, start)
+            length = np.zeros(rows, dtype=np.int32)
+            while True:
+                active = todo & ~seen[idx, x]
+                if not active.any():
+                    break
+                seen[idx[active], x[active]] = True
+                length[active] += 1
+                x = np.where(active, omegas[idx, x].astype(np.int64), x)
+            odd += (todo & (length % 2 == 1)).astype(np.int32)
     rate = np.float32(2) / (np.float32(max_moves) / np.float32(target))
 
     def put(field, values, dtype):

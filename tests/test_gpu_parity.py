@@ -192,3 +192,33 @@ def test_emitter_stream_certified_and_checked(O, native):
             sym = native.positions_to_symbols(beta, run.results[q].witness())
             assert certs.value(q) == O.codec_value(sym)
             assert certs.key(q, reqs[q].n) == O.codec_key(omega)
+
+
+def test_unoriented_stage_partitions_all_solved(O, native):
+    """The emitter's unoriented stage on every partition with at most 20,000
+    classes (32,880 searches of n = 17): each class has a witness of at most 9
+    moves (emitter.log reports residual=0 for every partition), each witness
+    replays and certifies, and a sample matches the oracle in the same order."""
+    from td16_amd import unoriented
+    cases = [(p, t) for p, t in unoriented.STAGE if native.bracelet_count(p) <= 20000]
+    sb = unoriented.stage_requests(cases)
+    assert sb.classes == [native.bracelet_count(p) for p, _ in cases]
+    count = len(sb.requests)
+    assert count == 32880
+    run = native.batch(sb.requests)
+    f = native.result_fields(run.results)
+    assert np.all(f["status"] == 0) and np.all(f["found"] == 1)
+    assert np.all((f["depth"] >= 1) & (f["depth"] <= 9))
+    certs = native.certify_batch(sb.requests, run.results)
+    assert np.all(certs.status == 0)
+    st = native.check_batch(*native.obligations(sb.requests), certs.values, certs.value_len)
+    assert np.all(st == 0)
+    for q in range(0, count, 97):
+        req = sb.requests[q]
+        beta, omega = pair_of(req)
+        ref = O.search(beta, omega, req.initial_even, req.min_rate, req.max_moves, True,
+                       greedy=True)
+        r = run.results[q]
+        assert ref.found == 1
+        if not r.split:
+            assert r.witness() == ref.witness() and r.expanded == ref.expanded

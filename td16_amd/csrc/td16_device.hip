@@ -882,17 +882,23 @@ __device__ void publish_result(const Params &P, Ctx &c, const WaveLds &w, uint32
     if (lane < 20) reinterpret_cast<uint32_t *>(&P.results[j])[lane] = result_word(P, c, w, j, lane);
     c.published++;  // added to the shared count once this wave runs out of work
   } else {
-    // record into the host ring slot of job slot j, then its index into the
-    // completion ring; the system fence orders the record before the index
+    // record into the host ring slot of job slot j and its index into the
+    // completion ring, then the sequence word the host polls: the host reads
+    // the index and the record only after it has acquired that word, so one
+    // system-scope release orders both before it
     if (lane < 20)
       reinterpret_cast<uint32_t *>(&P.hostResults[j])[lane] = result_word(P, c, w, j, lane);
-    __threadfence_system();
+    uint32_t cslot = 0, cnext = 0;
     if (lane == 0) {
       const uint64_t cseq = add_rlx(&P.ctrl->completions, (uint64_t)1);
-      const uint32_t cslot = (uint32_t)(cseq & (uint64_t)(P.ringCap - 1));
+      cslot = (uint32_t)(cseq & (uint64_t)(P.ringCap - 1));
+      cnext = (uint32_t)(cseq + 1);
       P.hostCompletionIdx[2 * cslot] = j;
-      __threadfence_system();
-      st_sys_rel(&P.hostCompletionIdx[2 * cslot + 1], (uint32_t)(cseq + 1));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    if (lane == 0) {
+      __hip_atomic_store(&P.hostCompletionIdx[2 * cslot + 1], cnext, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
       add_rlx(&P.ctrl->jobsDone, 1u);
     }
   }

@@ -349,6 +349,22 @@ int td16_bracelets(const int32_t *parts, const uint8_t *oriented, int m, int mod
                    int threads, uint8_t *out, int64_t cap_rows, int64_t *total,
                    char *msg, int msg_cap);
 
+/* The class of a fixed-point-free pair (TwistedBraceletKey.of,
+ * TwistedBraceletKey.java:22-29; canonicalRepresentative :96-98): beta (the
+ * cyclic sequence) and omega, permutations of 0..n-1, 2 <= n <= 32.  Writes
+ * the annotated type (*m cycles; parts[c], oriented[c] in the generator's
+ * order, longest first, unoriented before oriented; room for 16 entries),
+ * the class word (n symbols of the generator's alphabet; NULL to skip) and
+ * the class representative (omega's one-line notation for beta =
+ * (0 1 .. n-1), the row td16_bracelets emits for the class when its target
+ * is one n-cycle; NULL to skip).  A cycle is unoriented when omega maps each
+ * of its symbols to the cycle's previous symbol along beta.
+ * TD16_INVALID_ARGUMENT for non-permutations and for omega with a fixed
+ * point. */
+int td16_bracelet_canonical(const uint8_t *beta, const uint8_t *omega, int n,
+                            int32_t *parts, uint8_t *oriented, int32_t *m,
+                            uint8_t *word, uint8_t *rep_omega, char *msg, int msg_cap);
+
 /* Sustained int32 VALU rate of the device, in 10^12 lane-ops per second:
  * a grid of resident wavefronts runs independent dependent chains of
  * v_add_u32 / v_xor_b32 for `iterations` rounds.  Measurement aid for the

@@ -169,6 +169,7 @@ SIGNATURES = {
     "td16_check_batch": (_i, [_u8p, _u8p, _i32p, _i32p, _i, _u8p, _i32p, _i, _i32p, _c, _i]),
     "td16_bracelets": (_i, [_i32p, _u8p, _i, _i, _i, _u8p, ctypes.c_int64,
                             ctypes.POINTER(ctypes.c_int64), _c, _i]),
+    "td16_bracelet_canonical": (_i, [_u8p, _u8p, _i, _i32p, _u8p, _i32p, _u8p, _u8p, _c, _i]),
     "td16_gen_uniform": (_i, [ctypes.c_uint64, _i, _i, _u64p]),
     "td16_gen_products": (_i, [ctypes.c_uint64, _i, _i, _i, _i, _i, _u8p, _u8p, _u64p]),
     "td16_gen_exponential": (_i, [ctypes.c_uint64, _i, ctypes.POINTER(ctypes.c_double)]),

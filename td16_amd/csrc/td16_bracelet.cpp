@@ -299,12 +299,21 @@ class Generator {
       for (int p = 0; p < n; ++p) row[p] = (uint8_t)w_[p];
       return;
     }
+    decode_word(A_, w_, row);
+  }
+
+ public:
+  // omega of a word over positions 0..n-1 (DirectTwistedBraceletGenerator.java:
+  // 1475-1505): an unoriented cycle met at p1 < .. < pk maps p(i) to p(i-1)
+  // and p1 to pk; an oriented cycle maps its rank r's position to rank r+1's.
+  static void decode_word(const Alphabet &A, const int *w, uint8_t *row) {
+    const int n = A.n;
     int prev[kMaxParts], first[kMaxParts];
     int posByRank[kMaxBeads];
-    for (int c = 0; c < A_.m; ++c) prev[c] = first[c] = -1;
+    for (int c = 0; c < A.m; ++c) prev[c] = first[c] = -1;
     for (int p = 0; p < n; ++p) {
-      const int s = w_[p], c = A_.symCycle[s];
-      if (A_.oriented[c]) {
+      const int s = w[p], c = A.symCycle[s];
+      if (A.oriented[c]) {
         posByRank[s] = p;
         continue;
       }
@@ -312,15 +321,17 @@ class Generator {
       else first[c] = p;
       prev[c] = p;
     }
-    for (int c = 0; c < A_.m; ++c) {
-      if (A_.oriented[c]) {
-        const int base = A_.firstSym[c], sz = A_.size[c];
+    for (int c = 0; c < A.m; ++c) {
+      if (A.oriented[c]) {
+        const int base = A.firstSym[c], sz = A.size[c];
         for (int r = 0; r < sz; ++r) row[posByRank[base + r]] = (uint8_t)posByRank[base + (r + 1) % sz];
       } else {
         row[first[c]] = (uint8_t)prev[c];
       }
     }
   }
+
+ private:
 
   const Alphabet &A_;
   const bool pairs_;
@@ -460,5 +471,110 @@ extern "C" int td16_bracelets(const int32_t *parts, const uint8_t *oriented, int
                      "output holds %lld rows; %lld are needed", (long long)cap_rows,
                      (long long)n);
   }
+  return TD16_OK;
+}
+
+// The class of any fixed-point-free pair (TwistedBraceletKey.of, emitter/src/
+// main/java/br/unb/cic/tdp/base/TwistedBraceletKey.java:22-29): read omega
+// along beta's cyclic order, type its cycles (unoriented when omega maps each
+// position of the cycle to the cycle's previous position along beta), write
+// the word in the generator's alphabet and take the least standardized image
+// over all rotations and reflections.  The representative is that word
+// decoded, the pair the generator emits for the class when it is realizable.
+extern "C" int td16_bracelet_canonical(const uint8_t *beta, const uint8_t *omega, int n,
+                                       int32_t *parts, uint8_t *oriented, int32_t *m,
+                                       uint8_t *word, uint8_t *rep_omega, char *msg, int cap) {
+  using namespace td16;
+  if (beta == nullptr || omega == nullptr || m == nullptr || parts == nullptr ||
+      oriented == nullptr || n < 2 || n > kMaxBeads)
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid pair arguments (2 <= n <= %d)",
+                   kMaxBeads);
+  int pos[kMaxBeads];
+  bool seenB[kMaxBeads] = {}, seenO[kMaxBeads] = {};
+  for (int p = 0; p < n; ++p) {
+    if (beta[p] >= n || seenB[beta[p]] || omega[p] >= n || seenO[omega[p]])
+      return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "beta and omega must be permutations");
+    seenB[beta[p]] = seenO[omega[p]] = true;
+    pos[beta[p]] = p;
+  }
+  int om[kMaxBeads];  // omega on beta's positions
+  for (int p = 0; p < n; ++p) om[p] = pos[omega[beta[p]]];
+  int cycleOf[kMaxBeads], base[kMaxParts], nC = 0;
+  int32_t sz[kMaxParts];
+  uint8_t ori[kMaxParts];
+  for (int p = 0; p < n; ++p) cycleOf[p] = -1;
+  for (int p = 0; p < n; ++p) {
+    if (cycleOf[p] >= 0) continue;
+    if (om[p] == p)
+      return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
+                     "omega must be fixed-point free (every cycle of length at least two)");
+    int len = 0;
+    for (int x = p; cycleOf[x] < 0; x = om[x]) {
+      cycleOf[x] = nC;
+      ++len;
+    }
+    // the cycle's positions in increasing order: unoriented when each maps
+    // to the one before it, the first to the last
+    int prevPos = -1, firstPos = -1, lastPos = -1;
+    bool unori = true;
+    for (int x = 0; x < n; ++x) {
+      if (cycleOf[x] != nC) continue;
+      if (firstPos < 0) firstPos = x;
+      else if (om[x] != prevPos) unori = false;
+      prevPos = lastPos = x;
+    }
+    if (om[firstPos] != lastPos) unori = false;
+    base[nC] = p;
+    sz[nC] = len;
+    ori[nC] = unori ? 0 : 1;
+    ++nC;
+  }
+  Alphabet A;
+  const int st = make_alphabet(sz, ori, nC, &A, msg, cap);
+  if (st != TD16_OK) return st;
+  // each cycle takes a free slot of its annotated type; ranks count omega
+  // steps from the cycle's first position
+  int slotOf[kMaxParts];
+  bool taken[kMaxParts] = {};
+  for (int c = 0; c < nC; ++c) {
+    int s = 0;
+    while (taken[s] || A.size[s] != sz[c] || A.oriented[s] != (ori[c] != 0)) ++s;
+    taken[s] = true;
+    slotOf[c] = s;
+  }
+  int w0[kMaxBeads];
+  for (int c = 0; c < nC; ++c) {
+    int x = base[c];
+    for (int r = 0; r < sz[c]; ++r, x = om[x])
+      w0[x] = A.firstSym[slotOf[c]] + (ori[c] ? r : 0);
+  }
+  int best[kMaxBeads], cand[kMaxBeads];
+  bool have = false;
+  for (int s = 0; s < n; ++s) {
+    for (int dir = +1; dir >= -1; dir -= 2) {
+      Standardizer sd;
+      sd.reset(A.m);
+      bool smaller = !have, larger = false;
+      for (int k = 0; k < n && !larger; ++k) {
+        cand[k] = sd.next(A, w0[mod(s + dir * k, n)], dir);
+        if (!smaller) {
+          if (cand[k] < best[k]) smaller = true;
+          else if (cand[k] > best[k]) larger = true;
+        }
+      }
+      if (smaller) {
+        std::memcpy(best, cand, sizeof(int) * (size_t)n);
+        have = true;
+      }
+    }
+  }
+  *m = A.m;
+  for (int c = 0; c < A.m; ++c) {
+    parts[c] = A.size[c];
+    oriented[c] = A.oriented[c] ? 1 : 0;
+  }
+  if (word)
+    for (int p = 0; p < n; ++p) word[p] = (uint8_t)best[p];
+  if (rep_omega) Generator::decode_word(A, best, rep_omega);
   return TD16_OK;
 }

@@ -424,6 +424,33 @@ def bracelets(parts: Sequence[int], oriented: Optional[Sequence[bool]] = None,
         check(st, msg)
 
 
+@dataclass
+class BraceletClass:
+    """The class of a pair (TwistedBraceletKey.of): annotated type, class
+    word and representative pair (omega for beta = (0 1 .. n-1))."""
+    parts: List[int]
+    oriented: List[bool]
+    word: Tuple[int, ...]
+    representative: List[int]
+
+
+def bracelet_canonical(beta: Sequence[int], omega: Sequence[int]) -> BraceletClass:
+    n = len(beta)
+    if len(omega) != n:
+        raise Td16Error(_lib_mod.INVALID_ARGUMENT, "beta and omega differ in length")
+    parts = (ctypes.c_int32 * 16)()
+    ori = (ctypes.c_uint8 * 16)()
+    m = ctypes.c_int32(0)
+    word = (ctypes.c_uint8 * max(1, n))()
+    rep = (ctypes.c_uint8 * max(1, n))()
+    msg = msgbuf()
+    check(lib().td16_bracelet_canonical(_u8(beta), _u8(omega), n, parts, ori, ctypes.byref(m),
+                                        word, rep, msg, len(msg)), msg)
+    k = m.value
+    return BraceletClass(list(parts[:k]), [bool(x) for x in ori[:k]], tuple(word[:n]),
+                         list(rep[:n]))
+
+
 def bracelet_requests(omegas: np.ndarray, target: int, max_moves: int,
                       parts: Optional[Sequence[int]] = None) -> ctypes.Array:
     """Requests of the emitter's unoriented stage for representative pairs

@@ -147,3 +147,54 @@ def test_invalid_partitions(native):
     assert native.bracelet_count([2], [False]) == 0
     with pytest.raises(Td16Error):
         native.bracelet_count([20, 13])          # more than 32 beads
+
+
+def test_canonical_form_is_the_generators_row(native):
+    # canonicalRepresentative (TwistedBraceletKey.java:96-98) of each
+    # generated pair is that pair, also when beta starts elsewhere on its
+    # circle (a rotated word) and when the symbols are renamed along beta
+    rng = np.random.default_rng(7)
+    for parts, oriented in (([7, 4, 2, 2, 2], None), ([5, 3, 3], [True, False, False]),
+                            ([4, 3], [True, True]), ([6, 6, 5], None)):
+        rows = native.bracelets(parts, oriented)
+        words = native.bracelets(parts, oriented, words=True)
+        want = sorted(zip(parts, oriented or [False] * len(parts)), key=lambda a: (-a[0], a[1]))
+        word_set = {tuple(int(x) for x in w) for w in words}
+        for q in rng.choice(len(rows), size=min(200, len(rows)), replace=False):
+            omega = [int(x) for x in rows[q]]
+            n = len(omega)
+            cls = native.bracelet_canonical(list(range(n)), omega)
+            assert list(zip(cls.parts, cls.oriented)) == want
+            assert cls.representative == omega
+            assert cls.word in word_set
+            s = int(rng.integers(1, n))
+            beta = [(p + s) % n for p in range(n)]          # same cyclic sequence
+            omega2 = [0] * n
+            for x in range(n):                               # omega relabelled along beta
+                omega2[beta[x]] = beta[omega[x]]
+            assert native.bracelet_canonical(beta, omega2).representative == omega
+            assert native.bracelet_canonical(beta, omega).representative == omega
+
+
+def test_canonical_form_matches_brute_force_classes(native):
+    rng = np.random.default_rng(11)
+    done = 0
+    while done < 400:
+        n = int(rng.integers(4, 10))
+        omega = [int(x) for x in rng.permutation(n)]
+        if any(omega[x] == x for x in range(n)):
+            continue
+        beta = [int(x) for x in rng.permutation(n)]
+        # the same pair read along beta: positions instead of symbols
+        pos = {b: p for p, b in enumerate(beta)}
+        om = [pos[omega[beta[p]]] for p in range(n)]
+        key, word, A = B.annotated_type(om)
+        cls = native.bracelet_canonical(beta, omega)
+        assert tuple(sorted(zip(cls.parts, cls.oriented))) == key
+        assert cls.word == B.class_word(word, A)
+        assert cls.representative == B.decode(cls.word, A)
+        done += 1
+    with pytest.raises(Td16Error):
+        native.bracelet_canonical([0, 1, 2], [0, 2, 1])     # fixed point
+    with pytest.raises(Td16Error):
+        native.bracelet_canonical([0, 1, 1], [1, 2, 0])     # not a permutation

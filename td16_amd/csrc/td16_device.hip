@@ -107,14 +107,6 @@ __device__ __forceinline__ bool cas_rlx(T *p, T expected, T desired) {
 // never in polling loops.
 __device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
-template <class T>
-__device__ __forceinline__ T ld_sys_acq(T *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-template <class T>
-__device__ __forceinline__ void st_sys_rel(T *p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
 
 // Lane 0 runs f, every lane gets its int result.
 template <class F>
@@ -934,8 +926,20 @@ __device__ int start_job(const Params &P, Ctx &c, uint32_t j) {
   return limit;
 }
 
-// Served loop: copies a newly submitted request into the device job table.
-__device__ int take_submitted(const Params &P, uint32_t &jOut) {
+// Served loop: takes the next submitted request.  The submit ring and its
+// count live in host memory and are read only with atomic loads, which reach
+// host memory on every access (a probe that warmed the line with a plain
+// load still saw the host's next write within one ~1.2 us poll), so no cache
+// maintenance sits on this path: an agent-scope release or acquire here
+// would walk an XCD's L2, tens of microseconds under load, to protect data
+// that is never read through the cache.  One idle wavefront at a time (at
+// most every ~2 us by the chip's 100 MHz counter) copies the host's count
+// into a device-memory mirror; a claimer takes entry r with a CAS on
+// `consumed`, reads it over the bus, keeps it in jwOut (lane q < 32 holds word
+// q, as job_word returns it) and copies it to the job table for wavefronts
+// that may later receive subtrees of it (a donation's release orders those
+// stores before they are read).
+__device__ int take_submitted(const Params &P, uint32_t &jOut, uint32_t &jwOut) {
   const int lane = lane_id();
   int got = 0;
   uint64_t r = 0;
@@ -943,18 +947,16 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut) {
     uint64_t avail = ld_rlx(&P.ctrl->submittedMirror);
     r = ld_rlx(&P.ctrl->consumed);
     if (r >= avail) {
-      // refresh the mirror from host memory at most every ~2 us, timed by
-      // the chip-wide 100 MHz real-time counter
       const uint64_t now = __builtin_amdgcn_s_memrealtime();
       const uint64_t last = ld_rlx(&P.ctrl->lastPoll);
       if (now - last > 200ULL && cas_rlx(&P.ctrl->lastPoll, last, now)) {
-        // stop is read first: the host raises it only after its last submit
-        if (ld_sys_acq(&P.host->stop) != 0) st_rlx(&P.ctrl->stopMirror, (uint64_t)1);
-        avail = ld_sys_acq(&P.host->submitted);
-        // publish what that acquire made visible before advancing the mirror
-        fence_rel();
+        // one read gives the count and the stop bit together
+        const uint64_t h =
+            __hip_atomic_load(&P.host->submitted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (h & kServeStop) st_rlx(&P.ctrl->stopMirror, (uint64_t)1);
+        const uint64_t cnt = h & ~kServeStop;
         uint64_t cur = ld_rlx(&P.ctrl->submittedMirror);
-        while (cur < avail && !cas_rlx(&P.ctrl->submittedMirror, cur, avail))
+        while (cur < cnt && !cas_rlx(&P.ctrl->submittedMirror, cur, cnt))
           cur = ld_rlx(&P.ctrl->submittedMirror);
       }
       avail = ld_rlx(&P.ctrl->submittedMirror);
@@ -970,8 +972,6 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut) {
   got = uni(got);
   if (!got) return 0;
   r = uni64(r);
-  // pairs with the poller's release: entry r is visible from here on
-  fence_acq();
   // one read of the 128-byte record over the bus, 16 bytes per lane; ring
   // entries are transport only and the record's tag names its job slot
   const uint32_t e = (uint32_t)(r & (uint64_t)(P.ringCap - 1));
@@ -988,9 +988,14 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) dst[4 * lane + q] = v[q];
   }
-  // the record is re-read by this wave only (lanes other than its writer):
-  // workgroup scope suffices; a later donation's release publishes it further
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+  // lane q < 32 takes word q: element q % 4 of lane q / 4
+  const int from = (lane & 31) >> 2, sel = lane & 3;
+  const uint32_t w0 = (uint32_t)bperm(from, (int)v[0]);
+  const uint32_t w1 = (uint32_t)bperm(from, (int)v[1]);
+  const uint32_t w2 = (uint32_t)bperm(from, (int)v[2]);
+  const uint32_t w3 = (uint32_t)bperm(from, (int)v[3]);
+  const uint32_t wq = sel == 0 ? w0 : sel == 1 ? w1 : sel == 2 ? w2 : w3;
+  jwOut = lane < 32 ? wq : 0u;
   jOut = j;
   return 1;
 }
@@ -1025,6 +1030,7 @@ td16_search_kernel(Params P) {
   for (;;) {
     Item it;
     uint32_t j = 0;
+    uint32_t servedJw = 0;
     // While the batched feed lasts no wavefront idles, so the balance
     // counters hold the number of queued items: one round trip over them
     // spares the probes of an empty queue.
@@ -1059,7 +1065,7 @@ td16_search_kernel(Params P) {
           haveJob = true;
         }
       } else {
-        haveJob = take_submitted(P, j) != 0;
+        haveJob = take_submitted(P, j, servedJw) != 0;
       }
     }
     if (!haveItem && !haveJob) {
@@ -1119,7 +1125,7 @@ td16_search_kernel(Params P) {
         }
         if (lane < kBalanceShards) balv = ld_rlx(&P.ctrl->balance[lane].v);
       }
-      c.jw = job_word(P, j);
+      c.jw = P.served ? servedJw : job_word(P, j);
       if (!P.served && !c.drained) {
         int64_t bal = 0;
 #pragma unroll

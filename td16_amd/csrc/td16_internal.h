@@ -146,7 +146,7 @@ struct alignas(256) Control {
   uint64_t completions;  // completion-ring entries written
   uint64_t lastPoll;     // device clock of the last host-ring poll
   uint64_t submittedMirror;
-  uint64_t stopMirror;   // device copy of ServeHost::stop
+  uint64_t stopMirror;   // device copy of ServeHost::submitted's stop bit
   uint64_t pad5[27];
   FeedShard feed[kFeedShards];
 };
@@ -157,11 +157,12 @@ size_t init_feed(Control *c, uint32_t numJobs);
 
 // Host-visible rings of the served loop (fine-grained pinned host memory).
 struct alignas(128) ServeHost {
-  uint64_t submitted;    // entries published by the host
-  uint64_t pad0[15];
-  uint32_t stop;         // host asks the kernel to exit once idle
-  uint32_t pad1[31];
+  // entries published by the host; bit 63 (kServeStop), set only once every
+  // submitted request has completed, asks the kernel to exit
+  uint64_t submitted;
+  uint64_t pad0[31];
 };
+constexpr uint64_t kServeStop = 1ULL << 63;
 
 struct Params {
   const Job *jobs;

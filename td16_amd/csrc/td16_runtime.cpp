@@ -941,7 +941,7 @@ int td16_serve_stop(char *msg, int cap) {
     S->cvSpace.notify_all();
     S->cvDone.wait(l2, [&] { return S->waiting.empty() && S->inFlight == 0; });
   }
-  __atomic_store_n(&S->hHost->stop, 1u, __ATOMIC_RELEASE);
+  __atomic_store_n(&S->hHost->submitted, S->submittedSeq | kServeStop, __ATOMIC_RELEASE);
   hipSetDevice(S->device);
   hipError_t e = hipStreamSynchronize(S->stream);
   S->pollerExit.store(true, std::memory_order_release);

@@ -224,8 +224,10 @@ enum { kExpandOk = 0, kExpandFound = 1 };
 
 // Nodes after which an item counts as heavy: it gets raised wave priority
 // and, while unstarted requests remain, keeps up to kEarlyStock of its
-// subtrees queued.
+// subtrees queued.  Requests without slack (kFlagTight) are where long
+// searches come from and qualify sooner.
 constexpr int kHeavyNodes = 64;
+constexpr int kHeavyNodesSlack = 512;
 constexpr int kEarlyStock = 1024;
 
 // Packs the witness into the job state when this wave is the first finder.
@@ -755,7 +757,8 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
       // an item that has proved heavy gets the SIMD's issue priority over
       // the light requests sharing it, so it does not become the launch's
       // tail; light work fills the remaining issue slots
-      if (c.expanded >= (uint64_t)kHeavyNodes) __builtin_amdgcn_s_setprio(2);
+      const uint64_t heavyAt = (c.flags & kFlagTight) ? kHeavyNodes : kHeavyNodesSlack;
+      if (c.expanded >= heavyAt) __builtin_amdgcn_s_setprio(2);
       const int want = lane0([&] {
         // balance = queued items - idle wavefronts
         int64_t bal = 0;
@@ -768,7 +771,7 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
         // subtrees queued, which wavefronts take before new requests, so
         // heavy requests spread early instead of crawling on one wavefront
         // that shares its SIMD with light ones
-        int64_t room = drained ? -bal : (c.expanded >= (uint64_t)kHeavyNodes ? kEarlyStock - bal : 0);
+        int64_t room = drained ? -bal : (c.expanded >= heavyAt ? kEarlyStock - bal : 0);
         if (room <= 0) return 0;
         return room < (int64_t)P.donateMax ? (int)room : P.donateMax;
       });

@@ -183,6 +183,23 @@ int make_job_checked(const td16_request *r, Job *job, int *trivial, char *msg, i
   } else {
     rate_tables(N, r->initial_even, r->min_rate, r->max_moves, r->full_sorting, job->sat,
                 job->reach);
+    // No slack: the odd-cycle count rises by at most two per move, so depth
+    // d reaches initial_even .. initial_even + 2d; when the first depth with a
+    // witness among those is the move bound itself, every move must be
+    // perfect.  Long FIRST searches come from such requests.
+    if (r->mode == TD16_FIRST) {
+      for (int d = 0; d <= r->max_moves && d <= kMaxDepth; ++d) {
+        uint32_t odds = 0;  // odd-cycle counts depth d can reach
+        for (int i = 0; i <= d; ++i) {
+          const int e = r->initial_even + 2 * i;
+          if (e >= 0 && e < 32) odds |= 1u << e;
+        }
+        if (job->sat[d] & odds) {
+          if (d == r->max_moves) job->flags |= kFlagTight;
+          break;
+        }
+      }
+    }
   }
   return TD16_OK;
 }

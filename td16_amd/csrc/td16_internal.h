@@ -30,6 +30,8 @@ enum JobFlags : uint8_t {
   kFlagCountAll = 2,   // no early exit
   kFlagIddfs = 4,      // iterative deepening on the depth limit
   kFlagLexOrder = 8,   // FIRST search keeps the lexicographic order throughout
+  kFlagTight = 16,     // FIRST search without slack: the move bound is the first
+                       // depth at which a witness is possible (a long search)
 };
 
 // One request as the kernel reads it (host-written, 128 B).

@@ -195,23 +195,6 @@ void decode(const DevResult &r, td16_result *out) {
   unpack_moves(r.moves, out->depth, out->moves);
 }
 
-// The move bound when it is also the first depth at which a witness is
-// possible (the odd-cycle count can rise by at most two per move, so depth d
-// reaches odd0 .. odd0 + 2d), else -1; -1 for COUNT_ALL and IDDFS requests.
-int tight_depth(const Job &job, const td16_request &req) {
-  if (job.flags & (kFlagCountAll | kFlagIddfs)) return -1;
-  const int odd0 = req.initial_even;
-  for (int d = 0; d <= job.maxDepth && d < 10; ++d) {
-    uint32_t reachable = 0;
-    for (int i = 0; i <= d; ++i) {
-      const int e = odd0 + 2 * i;
-      if (e >= 0 && e < 32) reachable |= 1u << e;
-    }
-    if (job.sat[d] & reachable) return d == job.maxDepth ? d : -1;
-  }
-  return -1;
-}
-
 void trivial_result(int trivial, td16_result *out) {
   std::memset(out, 0, sizeof(*out));
   out->status = TD16_OK;
@@ -627,13 +610,12 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
   // feed shards (contiguous ranges of the table, see init_feed), so every
   // shard runs its own longest-first sequence.
   {
-    std::vector<int8_t> tight(nj);
-    for (uint32_t q = 0; q < nj; ++q) tight[q] = (int8_t)tight_depth(jobs[q], reqs[where[q]]);
     std::vector<uint32_t> order(nj);
     for (uint32_t q = 0; q < nj; ++q) order[q] = q;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
       const Job &x = jobs[a], &y = jobs[b];
-      if (tight[a] != tight[b]) return tight[a] > tight[b];
+      const bool tx = (x.flags & kFlagTight) != 0, ty = (y.flags & kFlagTight) != 0;
+      if (tx != ty) return tx;
       return x.maxDepth != y.maxDepth ? x.maxDepth > y.maxDepth : x.N > y.N;
     });
     const uint32_t per = (nj + kFeedShards - 1) / kFeedShards;

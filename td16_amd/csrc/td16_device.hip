@@ -225,8 +225,11 @@ enum { kExpandOk = 0, kExpandFound = 1 };
 // Nodes after which an item counts as heavy: it gets raised wave priority
 // and, while unstarted requests remain, keeps up to kEarlyStock of its
 // subtrees queued.  Requests without slack (kFlagTight) are where long
-// searches come from and qualify sooner.
+// searches come from and qualify sooner; with a move bound of 8 or more
+// (tens of thousands of nodes on average) at the first check.
 constexpr int kHeavyNodes = 64;
+constexpr int kHeavyNodesDeep = 0;
+constexpr int kDeepBound = 8;
 constexpr int kHeavyNodesSlack = 512;
 constexpr int kEarlyStock = 1024;
 
@@ -757,7 +760,9 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
       // an item that has proved heavy gets the SIMD's issue priority over
       // the light requests sharing it, so it does not become the launch's
       // tail; light work fills the remaining issue slots
-      const uint64_t heavyAt = (c.flags & kFlagTight) ? kHeavyNodes : kHeavyNodesSlack;
+      const uint64_t heavyAt =
+          !(c.flags & kFlagTight) ? kHeavyNodesSlack
+          : job_max_depth(c) >= kDeepBound ? kHeavyNodesDeep : kHeavyNodes;
       if (c.expanded >= heavyAt) __builtin_amdgcn_s_setprio(2);
       const int want = lane0([&] {
         // balance = queued items - idle wavefronts

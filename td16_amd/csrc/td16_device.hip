@@ -516,13 +516,22 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   if (lane == 0) {
     JobState *st = &P.states[c.job];
     if (!c.everShared) {
-      // first donation: the counts of finished IDDFS rounds, kept in
-      // registers so far, join the shared state (published by the fence)
+      // first donation: the request's shared state is written only now,
+      // from registers (a FIRST search still running has no witness yet;
+      // the counts of finished IDDFS rounds join here), and the fence below
+      // publishes it with the items
+      st->pending = 1 + cnt;
+      st->found = 0;
+      st->limit = c.limit;
+      st->split = cnt;
       st->expanded = c.totExpanded;
       st->solutions = c.totSolutions;
+      st->depth = 0;
+      st->t0 = c.t0;
+    } else {
+      add_rlx(&st->pending, cnt);
+      add_rlx(&st->split, cnt);
     }
-    add_rlx(&st->pending, cnt);
-    add_rlx(&st->split, cnt);
     add_rlx(&P.ctrl->balance[c.home % kBalanceShards].v, (int64_t)cnt);
   }
   c.everShared = true;
@@ -906,8 +915,7 @@ __device__ void publish_result(const Params &P, Ctx &c, const WaveLds &w, uint32
 }
 
 // Starts request j (root item) in this wave; returns its first depth limit.
-__device__ int start_job(const Params &P, Ctx &c, uint32_t j) {
-  const int lane = lane_id();
+__device__ int start_job(Ctx &c) {
   const uint32_t w7 = job_w7(c);
   const int limit = (int)((((w7 >> 16) & kFlagIddfs) ? (w7 >> 24) : (w7 >> 8)) & 0xFFu);
   c.t0 = __builtin_amdgcn_s_memrealtime();
@@ -916,21 +924,8 @@ __device__ int start_job(const Params &P, Ctx &c, uint32_t j) {
   c.totSolutions = 0;
   c.found = 0;
   c.wdepth = 0;
-  if (lane == 0) {
-    // read by other wavefronts only once this wave donates
-    JobState *st = &P.states[j];
-    st->pending = 1;
-    st->found = 0;
-    st->limit = limit;
-    st->split = 0;
-    st->expanded = 0;
-    st->solutions = 0;
-    st->depth = 0;
-    st->t0 = c.t0;
-  }
-  // no fence: until this wave donates (which releases), no other wavefront
-  // reads this state
-  wave_sync();
+  // the request's state in HBM is written at its first donation, if any:
+  // until then no other wavefront reads it, and most requests never donate
   return limit;
 }
 
@@ -1142,7 +1137,7 @@ td16_search_kernel(Params P) {
                            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)balv, q));
         c.stockHint = bal > 0 ? 1 : 0;
       }
-      limit = start_job(P, c, j);
+      limit = start_job(c);
       pi = job_pi0(c);
       depth0 = 0;
       c.shared = false;

@@ -200,7 +200,7 @@ class Generator {
     if (t == A_.n) {
       if (!complete_images(fwd, bwd)) return;
       ++*count;
-      if (out) emit(*out);
+      if (out && (stored_ == nullptr || stored_->fetch_add(1) < storeCap_)) emit(*out);
       return;
     }
     for (int s = 0; s < A_.S; ++s) {
@@ -210,6 +210,13 @@ class Generator {
       run(t + 1, f, b, out, count);
       unplace(t, s, U);
     }
+  }
+
+  // Rows beyond `cap` over all generators sharing `stored` are counted but
+  // not kept, so a too-small output cannot make the host memory grow.
+  void limit_storage(std::atomic<int64_t> *stored, int64_t cap) {
+    stored_ = stored;
+    storeCap_ = cap;
   }
 
   // Prefixes of length `depth` that survive every cut, in increasing order.
@@ -343,6 +350,8 @@ class Generator {
   int other_[kMaxBeads];
   int arcs_;
   uint32_t met_;
+  std::atomic<int64_t> *stored_ = nullptr;
+  int64_t storeCap_ = 0;
 };
 
 int make_alphabet(const int32_t *parts, const uint8_t *oriented, int m, Alphabet *A, char *msg,
@@ -431,10 +440,12 @@ extern "C" int td16_bracelets(const int32_t *parts, const uint8_t *oriented, int
   }
   std::vector<std::vector<uint8_t>> rows;
   std::vector<int64_t> counts;
+  std::atomic<int64_t> stored{0};
   if (nt == 1 || depth >= A.n || nPrefix == 0) {
     rows.resize(1);
     counts.assign(1, 0);
     Generator g(A, pairs);
+    g.limit_storage(&stored, cap_rows);
     g.run(0, 0, 0, keep ? &rows[0] : nullptr, &counts[0]);
   } else {
     rows.resize((size_t)nPrefix);
@@ -445,6 +456,7 @@ extern "C" int td16_bracelets(const int32_t *parts, const uint8_t *oriented, int
         const int q = next.fetch_add(1);
         if (q >= nPrefix) return;
         Generator g(A, pairs);
+        g.limit_storage(&stored, cap_rows);
         uint32_t f = 0, b = 0;
         Undo U[kMaxBeads];
         for (int t = 0; t < depth; ++t) g.place(t, prefixes[(size_t)q * depth + t], f, b, U[t]);

@@ -198,3 +198,20 @@ def test_canonical_form_matches_brute_force_classes(native):
         native.bracelet_canonical([0, 1, 2], [0, 2, 1])     # fixed point
     with pytest.raises(Td16Error):
         native.bracelet_canonical([0, 1, 1], [1, 2, 0])     # not a permutation
+
+
+def test_small_output_buffer_reports_the_count(native):
+    import ctypes
+    parts = (ctypes.c_int32 * 5)(6, 4, 3, 2, 2)
+    ori = (ctypes.c_uint8 * 5)()
+    for threads in (1, 4):
+        out = np.zeros((10, 17), np.uint8)
+        total = ctypes.c_int64(0)
+        msg = _lib.msgbuf()
+        st = native.lib().td16_bracelets(parts, ori, 5, _lib.BRACELET_PAIRS, threads,
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                         10, ctypes.byref(total), msg, len(msg))
+        assert st == _lib.INVALID_ARGUMENT and total.value == 119000
+        assert b"119000" in msg.value
+        rows = native.bracelets([6, 4, 3, 2, 2])          # the wrapper retries with room
+        assert {bytes(r) for r in out} <= {bytes(r) for r in rows}

@@ -1098,10 +1098,10 @@ td16_search_kernel(Params P) {
       }
       if (done) break;
       // idle polling touches a few words of HBM; back off to keep those
-      // cache lines cool (about 0.4 us per unit, up to ~14 us batched and
-      // ~3.5 us served)
+      // cache lines cool (about 0.4 us per unit, up to ~3.5 us served and
+      // P.idleBackoff units batched)
       for (int q = 0; q < backoff; ++q) __builtin_amdgcn_s_sleep(16);
-      const int maxBackoff = P.served ? 8 : 32;
+      const int maxBackoff = P.served ? 8 : P.idleBackoff;
       backoff = backoff < maxBackoff ? backoff * 2 : maxBackoff;
       continue;
     }

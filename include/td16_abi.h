@@ -192,6 +192,35 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
                const td16_batch_options *opt, td16_batch_stats *stats,
                char *msg, int msg_cap);
 
+/* One configuration the autotuner timed. */
+typedef struct {
+    int32_t waves_per_cu;
+    int32_t split_depth;
+    int32_t donate_max;
+    int32_t waves;           /* wavefronts in the grid it gave */
+    double kernel_ms;        /* median device time of one launch */
+} td16_tune_point;
+
+/* Autotuner for the batched launch: picks waves_per_cu (wavefronts per CU),
+ * split_depth (how far above the move bound a DFS frame may still be
+ * donated: the stack depth at which work is shared) and donate_max (the
+ * steal granularity) for this batch by timing it.  Coordinate descent from
+ * `base` over waves_per_cu {4, 8, 12, 16}, split_depth {2..7} and
+ * donate_max {4, 8, 16, 32, 64}; each point is one untimed launch and
+ * `repeats` timed ones (median), and a point must beat the current choice by
+ * more than 1 % to replace it.  The knobs only change how work is shared:
+ * every point's answers (found; the distance of iterative-deepening
+ * requests; the counts of COUNT_ALL requests) must equal the first point's,
+ * else TD16_DEVICE_ERROR.  On success `best` is `base` with the three knobs
+ * set, results[q] holds request q's result from the first point and
+ * points[0 .. min(*npoints, points_cap)) the timed configurations in the
+ * order tried.  The reference picks none of these (its queue count comes
+ * from the occupancy API, sorting_search.cu:714-752). */
+int td16_autotune(const td16_request *reqs, int count, td16_result *results,
+                  const td16_batch_options *base, int repeats,
+                  td16_batch_options *best, td16_tune_point *points,
+                  int points_cap, int *npoints, char *msg, int msg_cap);
+
 /* ---------------------------------------------------------------------- */
 /* Served loop: continuous batching into a running persistent kernel       */
 /* ---------------------------------------------------------------------- */

@@ -137,6 +137,16 @@ class ServeStats(ctypes.Structure):
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
+class TunePoint(ctypes.Structure):
+    _fields_ = [
+        ("waves_per_cu", ctypes.c_int32),
+        ("split_depth", ctypes.c_int32),
+        ("donate_max", ctypes.c_int32),
+        ("waves", ctypes.c_int32),
+        ("kernel_ms", ctypes.c_double),
+    ]
+
+
 _c = ctypes.c_char_p
 _i = ctypes.c_int
 SIGNATURES = {
@@ -148,6 +158,9 @@ SIGNATURES = {
     "td16_search_ex": (_i, [_i, ctypes.POINTER(Request), ctypes.POINTER(Result), _c, _i]),
     "td16_batch": (_i, [ctypes.POINTER(Request), _i, ctypes.POINTER(Result),
                         ctypes.POINTER(BatchOptions), ctypes.POINTER(BatchStats), _c, _i]),
+    "td16_autotune": (_i, [ctypes.POINTER(Request), _i, ctypes.POINTER(Result),
+                           ctypes.POINTER(BatchOptions), _i, ctypes.POINTER(BatchOptions),
+                           ctypes.POINTER(TunePoint), _i, _i32p, _c, _i]),
     "td16_serve_start": (_i, [ctypes.POINTER(ServeOptions), _c, _i]),
     "td16_serve_submit": (_i, [ctypes.POINTER(Request), _i, _u64p, _c, _i]),
     "td16_serve_wait": (_i, [ctypes.c_uint64, ctypes.POINTER(Result), _c, _i]),

@@ -569,21 +569,27 @@ int td16_search(int slot, const uint8_t *beta, const uint8_t *omega, int n, int 
   return TD16_OK;
 }
 
-int td16_batch(const td16_request *reqs, int count, td16_result *results,
-               const td16_batch_options *opt, td16_batch_stats *stats, char *msg, int cap) {
-  if (count < 0 || (count > 0 && (reqs == nullptr || results == nullptr)))
-    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid batch arguments");
+}  // extern "C"
+
+namespace td16 {
+namespace {
+
+td16_batch_options normalize_batch_options(const td16_batch_options *opt) {
   td16_batch_options o;
   std::memset(&o, 0, sizeof(o));
   if (opt) o = *opt;
   if (o.repeats < 1) o.repeats = 1;
   if (o.queue_bytes <= 0) o.queue_bytes = 256LL << 20;
-  td16_batch_stats localStats;
-  std::memset(&localStats, 0, sizeof(localStats));
+  return o;
+}
 
-  // host side: validate and relabel every request
-  std::vector<Job> jobs;
-  std::vector<int> where;
+// Host side of a batch: validates and relabels every request, answers the
+// trivial ones in `results`, and returns the rest in feed order (`where[q]`
+// is the caller's index of jobs[q]).
+int prepare_batch(const td16_request *reqs, int count, td16_result *results,
+                  std::vector<Job> &jobs, std::vector<int> &where, char *msg, int cap) {
+  jobs.clear();
+  where.clear();
   jobs.reserve((size_t)count);
   for (int q = 0; q < count; ++q) {
     Job job;
@@ -641,7 +647,17 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
     jobs.swap(dealtJobs);
     where.swap(dealtWhere);
   }
+  return TD16_OK;
+}
 
+// Device side of a batch: `o.repeats` persistent launches over the prepared
+// requests; results of the last one land in results[where[q]].
+int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
+              const td16_batch_options &o, td16_result *results, td16_batch_stats *stats,
+              char *msg, int cap) {
+  const uint32_t nj = (uint32_t)jobs.size();
+  td16_batch_stats localStats;
+  std::memset(&localStats, 0, sizeof(localStats));
   std::lock_guard<std::mutex> lk(gBatchMu);
   DeviceInfo dev;
   int st = device_info(o.device, &dev, msg, cap);
@@ -734,6 +750,147 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
     localStats.donations += (uint64_t)r.split;
   }
   if (stats) *stats = localStats;
+  return TD16_OK;
+}
+
+}  // namespace
+}  // namespace td16
+
+extern "C" {
+
+int td16_batch(const td16_request *reqs, int count, td16_result *results,
+               const td16_batch_options *opt, td16_batch_stats *stats, char *msg, int cap) {
+  if (count < 0 || (count > 0 && (reqs == nullptr || results == nullptr)))
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid batch arguments");
+  const td16_batch_options o = normalize_batch_options(opt);
+  std::vector<Job> jobs;
+  std::vector<int> where;
+  const int st = prepare_batch(reqs, count, results, jobs, where, msg, cap);
+  if (st != TD16_OK) return st;
+  return run_batch(jobs, where, o, results, stats, msg, cap);
+}
+
+int td16_autotune(const td16_request *reqs, int count, td16_result *results,
+                  const td16_batch_options *base, int repeats, td16_batch_options *best,
+                  td16_tune_point *points, int pointsCap, int *npoints, char *msg, int cap) {
+  if (count <= 0 || reqs == nullptr || results == nullptr || best == nullptr)
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
+                   "autotune needs a non-empty batch, its results and an output for the choice");
+  if (repeats < 1 || repeats > 100)
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "autotune repeats must be in 1..100");
+  if (pointsCap < 0 || (pointsCap > 0 && points == nullptr))
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid tuning-point buffer");
+  if (npoints) *npoints = 0;
+  td16_batch_options o = normalize_batch_options(base);
+  o.repeats = 1;
+  std::vector<Job> jobs;
+  std::vector<int> where;
+  int st = prepare_batch(reqs, count, results, jobs, where, msg, cap);
+  if (st != TD16_OK) return st;
+
+  // The knobs change how the work is shared, never the answer: whether a
+  // request has a witness, its exact distance under iterative deepening and
+  // the counts of an exhaustive search are the same at every point, and any
+  // difference is reported as a device error.
+  // trivial requests are answered by prepare_batch; run_batch rewrites the rest
+  std::vector<td16_result> first, cur(results, results + count);
+  auto same_answers = [&](const std::vector<td16_result> &a, const std::vector<td16_result> &b) {
+    for (int q = 0; q < count; ++q) {
+      if (a[q].found != b[q].found) return false;
+      if (reqs[q].iddfs && a[q].depth != b[q].depth) return false;
+      if (reqs[q].mode == TD16_COUNT_ALL &&
+          (a[q].expanded != b[q].expanded || a[q].solutions != b[q].solutions))
+        return false;
+    }
+    return true;
+  };
+
+  struct Point {
+    int w, s, d;
+    double ms;
+    int waves;
+  };
+  std::vector<Point> seen;
+  // one untimed launch, then `repeats` timed ones; the median device time
+  auto measure = [&](int w, int sd, int dm, double *ms) -> int {
+    for (const Point &p : seen)
+      if (p.w == w && p.s == sd && p.d == dm) {
+        *ms = p.ms;
+        return TD16_OK;
+      }
+    o.waves_per_cu = w;
+    o.split_depth = sd;
+    o.donate_max = dm;
+    td16_batch_stats bs;
+    int rc = run_batch(jobs, where, o, cur.data(), &bs, msg, cap);
+    if (rc != TD16_OK) return rc;
+    std::vector<double> t;
+    for (int r = 0; r < repeats; ++r) {
+      rc = run_batch(jobs, where, o, cur.data(), &bs, msg, cap);
+      if (rc != TD16_OK) return rc;
+      t.push_back(bs.kernel_ms);
+    }
+    if (first.empty()) {
+      first = cur;
+    } else if (!same_answers(first, cur)) {
+      return set_msg(msg, cap, TD16_DEVICE_ERROR,
+                     "autotune: waves_per_cu=%d split_depth=%d donate_max=%d changed a result", w,
+                     sd, dm);
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    seen.push_back(Point{w, sd, dm, *ms, bs.waves});
+    return TD16_OK;
+  };
+
+  static const int kWaves[] = {4, 8, 12, 16};
+  static const int kSplit[] = {2, 3, 4, 5, 6, 7};
+  static const int kDonate[] = {4, 8, 16, 32, 64};
+  int bw = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
+  int bsd = o.split_depth > 0 ? o.split_depth : 5;
+  int bdm = o.donate_max > 0 ? std::min(o.donate_max, 64) : 64;
+  double bms = 0.0;
+  st = measure(bw, bsd, bdm, &bms);
+  if (st != TD16_OK) return st;
+  // coordinate descent, one knob at a time, until a full pass changes nothing
+  for (int pass = 0; pass < 3; ++pass) {
+    bool moved = false;
+    for (int knob = 0; knob < 3; ++knob) {
+      const int *vals = knob == 0 ? kWaves : knob == 1 ? kSplit : kDonate;
+      const int nv = knob == 0 ? 4 : knob == 1 ? 6 : 5;
+      for (int v = 0; v < nv; ++v) {
+        const int w = knob == 0 ? vals[v] : bw;
+        const int sd = knob == 1 ? vals[v] : bsd;
+        const int dm = knob == 2 ? vals[v] : bdm;
+        double ms = 0.0;
+        st = measure(w, sd, dm, &ms);
+        if (st != TD16_OK) return st;
+        // a new point must win by more than 1 % to displace the current one
+        if (ms < bms * 0.99) {
+          bw = w;
+          bsd = sd;
+          bdm = dm;
+          bms = ms;
+          moved = true;
+        }
+      }
+    }
+    if (!moved) break;
+  }
+  *best = normalize_batch_options(base);
+  best->waves_per_cu = bw;
+  best->split_depth = bsd;
+  best->donate_max = bdm;
+  for (int q = 0; q < count; ++q) results[q] = first[q];
+  const int np = (int)seen.size();
+  for (int q = 0; q < np && q < pointsCap; ++q) {
+    points[q].waves_per_cu = seen[q].w;
+    points[q].split_depth = seen[q].s;
+    points[q].donate_max = seen[q].d;
+    points[q].waves = seen[q].waves;
+    points[q].kernel_ms = seen[q].ms;
+  }
+  if (npoints) *npoints = np;
   return TD16_OK;
 }
 

@@ -16,7 +16,8 @@ import numpy as np
 
 from . import _lib as _lib_mod
 from ._lib import (FIRST, COUNT_ALL, MOVES_ALL, MOVES_REF, BatchOptions, BatchStats,
-                   Request, Result, ServeOptions, ServeStats, Td16Error, check, lib, msgbuf)
+                   Request, Result, ServeOptions, ServeStats, Td16Error, TunePoint, check, lib,
+                   msgbuf)
 
 Move = Tuple[int, int, int]
 
@@ -131,6 +132,37 @@ def batch(reqs: Sequence[Request], device: int = 0, waves_per_cu: int = 0, split
     m = msgbuf()
     check(lib().td16_batch(arr, n, out, ctypes.byref(opt), ctypes.byref(st), m, len(m)), m)
     return BatchRun(out, st)
+
+
+@dataclass
+class Tuning:
+    best: dict            # waves_per_cu, split_depth, donate_max
+    points: List[dict]    # every timed configuration, in the order tried
+    results: ctypes.Array
+
+
+def autotune(reqs: Sequence[Request], device: int = 0, repeats: int = 3, use_graph: bool = True,
+             queue_bytes: int = 0, waves_per_cu: int = 0, split_depth: int = 0,
+             donate_max: int = 0, max_points: int = 64) -> Tuning:
+    """Times the batch over the knob grid of ``td16_autotune`` and returns the
+    fastest setting; pass ``best`` to :func:`batch`.  Raises ``Td16Error`` if
+    any setting changed an answer."""
+    n = len(reqs)
+    arr = (Request * max(1, n))(*reqs) if not isinstance(reqs, ctypes.Array) else reqs
+    out = (Result * max(1, n))()
+    base = BatchOptions(device, waves_per_cu, split_depth, donate_max, 1 if use_graph else 0, 1,
+                        queue_bytes)
+    best = BatchOptions()
+    pts = (TunePoint * max_points)()
+    npts = ctypes.c_int32(0)
+    m = msgbuf()
+    check(lib().td16_autotune(arr, n, out, ctypes.byref(base), repeats, ctypes.byref(best), pts,
+                              max_points, ctypes.byref(npts), m, len(m)), m)
+    points = [dict(waves_per_cu=p.waves_per_cu, split_depth=p.split_depth,
+                   donate_max=p.donate_max, waves=p.waves, kernel_ms=p.kernel_ms)
+              for p in pts[:min(npts.value, max_points)]]
+    return Tuning(dict(waves_per_cu=best.waves_per_cu, split_depth=best.split_depth,
+                       donate_max=best.donate_max), points, out)
 
 
 # ---------------------------------------------------------------------------

@@ -222,3 +222,52 @@ def test_unoriented_stage_partitions_all_solved(O, native):
         assert ref.found == 1
         if not r.split:
             assert r.witness() == ref.witness() and r.expanded == ref.expanded
+
+
+# Settings of the three sharing knobs (wavefronts per CU, donation depth,
+# steal granularity), from the sparsest grid to the densest.
+KNOB_POINTS = [dict(waves_per_cu=4, split_depth=2, donate_max=4),
+               dict(waves_per_cu=8, split_depth=7, donate_max=64),
+               dict(waves_per_cu=16, split_depth=3, donate_max=16),
+               dict(waves_per_cu=12, split_depth=5, donate_max=32)]
+KNOBS = ("waves_per_cu", "split_depth", "donate_max")
+
+
+def knob_requests(native):
+    """Exhaustive searches in both move sets plus a slice of the batched mix."""
+    ns, ks, perms = native.gen_products(5, 8, 11, 4, 6, 48)
+    reqs = [bounded_request(native, perms[q], ns[q], 0, ms, native.COUNT_ALL)
+            for q in range(48) for ms in (native.MOVES_REF, native.MOVES_ALL)]
+    ns, ks, perms = native.gen_products(1, 12, 16, 4, 9, 600)
+    reqs += [native.linear_request(int(perms[q]), int(ns[q]), int(ks[q])) for q in range(600)]
+    return reqs
+
+
+def test_sharing_knobs_never_change_answers(O, native):
+    """The knobs decide how work is shared, not what is found: exhaustive
+    counts equal the oracle's at every setting, and found / not-found never
+    moves."""
+    reqs = knob_requests(native)
+    runs = [native.batch(reqs, **k) for k in KNOB_POINTS]
+    assert sum(r.stats.donations for r in runs) > 0
+    for q, req in enumerate(reqs):
+        for run in runs:
+            check_against_oracle(O, native, req, run.results[q])
+
+
+def test_autotune_picks_a_timed_point_and_keeps_answers(O, native):
+    reqs = knob_requests(native)
+    t = native.autotune(reqs, repeats=1)
+    assert tuple(t.points[0][k] for k in KNOBS) == (16, 5, 64)  # the library defaults
+    # the first pass times every value of every knob once
+    assert len(t.points) >= 4 + 5 + 4
+    chosen = [p for p in t.points if all(p[k] == t.best[k] for k in KNOBS)]
+    assert len(chosen) == 1
+    assert chosen[0]["kernel_ms"] <= t.points[0]["kernel_ms"]
+    # wavefronts per CU as asked: the grid scales with the knob
+    assert len({p["waves"] // p["waves_per_cu"] for p in t.points}) == 1
+    for q, req in enumerate(reqs):
+        check_against_oracle(O, native, req, t.results[q])
+    run = native.batch(reqs, **t.best)
+    for q, req in enumerate(reqs):
+        check_against_oracle(O, native, req, run.results[q])

@@ -338,3 +338,14 @@ def test_stats_window_reset_keeps_active_search():
     assert third.completed == 1 and third.started == 0
     with pytest.raises(RuntimeError):
         stats.released(0, 2200, False)
+
+
+def test_autotune_rejects_bad_arguments_before_any_launch(native):
+    req = native.make_request([0, 1, 2], [1, 2, 0], 1, 2.0, 1, True, native.MOVES_REF,
+                              native.FIRST)
+    with pytest.raises(Td16Error, match="non-empty batch"):
+        native.autotune([])
+    with pytest.raises(Td16Error, match="repeats"):
+        native.autotune([req], repeats=0)
+    with pytest.raises(Td16Error, match="repeats"):
+        native.autotune([req], repeats=101)

@@ -393,7 +393,7 @@ def main() -> int:
                         ("hipGraph-captured " if use_graph else ""),
             "instances": len(ns),
             "waves": st.waves,
-            "knobs": knobs or "library defaults (16 waves/CU, split depth 5, donate 64)",
+            "knobs": knobs or "library defaults (16 waves/CU, split depth 5, donate 16)",
         },
         "nodes_per_s": round(nodes_per_s, 1),
         "kernel_ms": round(st.kernel_ms, 4),

@@ -170,7 +170,7 @@ typedef struct {
     int32_t waves_per_cu;    /* resident wavefronts per CU; 0 = 16 (tuned) */
     int32_t split_depth;     /* frames deeper than max_moves - split_depth are
                                 never donated; 0 = default (5) */
-    int32_t donate_max;      /* children donated per donation; 0 = default (64) */
+    int32_t donate_max;      /* children donated per donation (at most 64); 0 = default (16) */
     int32_t use_graph;       /* 1 = launch through a captured hipGraph */
     int32_t repeats;         /* run the same batch this many times (>= 1) */
     int64_t queue_bytes;     /* HBM for the shared work queue; 0 = 256 MiB */

@@ -258,7 +258,7 @@ def test_sharing_knobs_never_change_answers(O, native):
 def test_autotune_picks_a_timed_point_and_keeps_answers(O, native):
     reqs = knob_requests(native)
     t = native.autotune(reqs, repeats=1)
-    assert tuple(t.points[0][k] for k in KNOBS) == (16, 5, 64)  # the library defaults
+    assert tuple(t.points[0][k] for k in KNOBS) == (16, 5, 16)  # the library defaults
     # the first pass times every value of every knob once
     assert len(t.points) >= 4 + 5 + 4
     chosen = [p for p in t.points if all(p[k] == t.best[k] for k in KNOBS)]

@@ -54,6 +54,26 @@ def build_requests(native, ns, ks, perms, moveset=0, mode=0):
     return arr
 
 
+def strided_sample(reqs, stride):
+    """Every `stride`-th request of a ctypes request array, as a new array."""
+    from td16_amd._lib import Request
+    size = ctypes.sizeof(Request)
+    raw = np.frombuffer(reqs, dtype=np.uint8).reshape(len(reqs), size)[::stride]
+    return (Request * len(raw)).from_buffer_copy(np.ascontiguousarray(raw).tobytes())
+
+
+def tune_on_sample(native, reqs, stride):
+    """Knobs for a large batch, tuned on every `stride`-th request of it."""
+    t0 = time.perf_counter()
+    sample = strided_sample(reqs, stride)
+    t = native.autotune(sample, repeats=1, use_graph=False)
+    info = {"knobs": t.best, "sample": len(sample), "points": len(t.points),
+            "sample_default_ms": round(t.points[0]["kernel_ms"], 3),
+            "sample_best_ms": round(min(p["kernel_ms"] for p in t.points), 3),
+            "seconds": round(time.perf_counter() - t0, 2)}
+    return t.best, info
+
+
 def host_threads():
     return max(1, min(16, os.cpu_count() or 1))
 
@@ -145,7 +165,7 @@ def cpu_baseline(native, ns, ks, perms, seconds_target=15.0):
     }
 
 
-def emitter_stream(native, count):
+def emitter_stream(native, count, tune=True):
     """The emitter-shaped stream without a JVM or RocksDB: `count` n = 16
     instances, depth bound 9 (9 in 10 products of 9 block transpositions, 1
     in 10 reverse-like), searched in one batched launch (after one untimed
@@ -153,8 +173,9 @@ def emitter_stream(native, count):
     records) and every stored record checked again by the batched checker."""
     ns, ks, perms = native.gen_emitter_stream(5, count)
     reqs = build_requests(native, ns, ks, perms)
-    native.batch(reqs, use_graph=False)
-    run = native.batch(reqs, use_graph=False)
+    knobs, tuning = tune_on_sample(native, reqs, 5) if tune else ({}, None)
+    native.batch(reqs, use_graph=False, **knobs)
+    run = native.batch(reqs, use_graph=False, **knobs)
     st = run.stats
     th = host_threads()
     t0 = time.perf_counter()
@@ -181,10 +202,11 @@ def emitter_stream(native, count):
         "witnesses_per_s_search": round(certified / (st.wall_ms * 1e-3), 1),
         "witnesses_per_s_search_plus_certify": round(
             certified / (st.wall_ms * 1e-3 + certify_s), 1),
+        "autotune": tuning,
     }
 
 
-def unoriented_stage(native):
+def unoriented_stage(native, tune=True):
     """The emitter's unoriented stage end to end: generate every realizable
     class of the 21 partitions of 17 (3,858,884 pairs), search them all in
     one batched launch (after one untimed launch of the same batch), certify
@@ -196,8 +218,10 @@ def unoriented_stage(native):
     sb = unoriented.stage_requests(threads=th)
     gen_s = time.perf_counter() - t0
     count = len(sb.requests)
-    native.batch(sb.requests, use_graph=False)
-    run = native.batch(sb.requests, use_graph=False)
+    # the requests come grouped by partition: a strided sample covers them all
+    knobs, tuning = tune_on_sample(native, sb.requests, 10) if tune else ({}, None)
+    native.batch(sb.requests, use_graph=False, **knobs)
+    run = native.batch(sb.requests, use_graph=False, **knobs)
     st = run.stats
     f = native.result_fields(run.results)
     t0 = time.perf_counter()
@@ -222,6 +246,7 @@ def unoriented_stage(native):
         "host_threads": th,
         "classes_per_s_end_to_end": round(
             certified / (gen_s + st.wall_ms * 1e-3 + certify_s), 1),
+        "autotune": tuning,
     }
 
 
@@ -357,10 +382,11 @@ def main() -> int:
         extra["certificates"] = certificate_rates(native, reqs, run.results)
         log("certificates", extra["certificates"])
     if "emitter" not in skip and args.emitter_instances > 0:
-        extra["emitter_stream"] = emitter_stream(native, args.emitter_instances)
+        extra["emitter_stream"] = emitter_stream(native, args.emitter_instances,
+                                                 tune="tune" not in skip)
         log("emitter stream", extra["emitter_stream"])
     if "stage" not in skip:
-        extra["unoriented_stage"] = unoriented_stage(native)
+        extra["unoriented_stage"] = unoriented_stage(native, tune="tune" not in skip)
         log("unoriented stage", extra["unoriented_stage"])
     cpu = None
     if "cpu" not in skip:

@@ -167,7 +167,8 @@ int td16_search_ex(int slot, const td16_request *req, td16_result *res,
 
 typedef struct {
     int32_t device;          /* HIP device ordinal */
-    int32_t waves_per_cu;    /* resident wavefronts per CU; 0 = 16 (tuned) */
+    int32_t waves_per_cu;    /* resident wavefronts per CU, in steps of 4; 0 = 16
+                                (tuned); 20 runs a 96-VGPR build of the kernel */
     int32_t split_depth;     /* frames deeper than max_moves - split_depth are
                                 never donated; 0 = default (5) */
     int32_t donate_max;      /* children donated per donation (at most 64); 0 = default (16) */
@@ -205,7 +206,7 @@ typedef struct {
  * split_depth (how far above the move bound a DFS frame may still be
  * donated: the stack depth at which work is shared) and donate_max (the
  * steal granularity) for this batch by timing it.  Coordinate descent from
- * `base` over waves_per_cu {4, 8, 12, 16}, split_depth {2..7} and
+ * `base` over waves_per_cu {4, 8, 12, 16, 20}, split_depth {2..7} and
  * donate_max {4, 8, 16, 32, 64}; each point is one untimed launch and
  * `repeats` timed ones (median), and a point must beat the current choice by
  * more than 1 % to replace it.  The knobs only change how work is shared:

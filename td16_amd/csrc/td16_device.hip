@@ -1003,14 +1003,9 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut, uint32_t &jwOut) 
   return 1;
 }
 
-}  // namespace
-
-// Register budget for four waves per SIMD (128 VGPRs): the few registers the
-// cap pushes to scratch belong to per-request set-up, not the per-node path,
-// and the measured mix runs fastest with it at three resident waves per SIMD.
-__global__ void __launch_bounds__(kBlockThreads, 4)
-td16_search_kernel(Params P) {
-  __shared__ BlockLds lds;
+// The persistent search loop of one block; the kernels below instantiate it
+// under two register budgets.
+__device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
   for (int q = threadIdx.x; q < kLexEntries; q += blockDim.x) lds.lex[q] = P.lex[q];
   __syncthreads();
 
@@ -1197,15 +1192,42 @@ td16_search_kernel(Params P) {
   }
 }
 
+}  // namespace
+
+// Register budget for four waves per SIMD (128 VGPRs): the few registers the
+// cap pushes to scratch belong to per-request set-up, not the per-node path.
+// The batched mix (about 6 nodes per request) runs fastest with it at 16
+// waves per CU.
+__global__ void __launch_bounds__(kBlockThreads, 4)
+td16_search_kernel(Params P) {
+  __shared__ BlockLds lds;
+  search_blocks(P, lds);
+}
+
+// Five waves per SIMD (96 VGPRs): the per-request paths spill about 40
+// registers to scratch, which costs the request-bound mix ~8 %, while
+// searches of tens of nodes per request gain from the fifth wave (the
+// emitter-shaped stream: -6 % at 20 waves per CU).  Chosen per launch by the
+// wavefronts-per-CU knob.
+__global__ void __launch_bounds__(kBlockThreads, 5)
+td16_search_kernel_w5(Params P) {
+  __shared__ BlockLds lds;
+  search_blocks(P, lds);
+}
+
 // Every shard's slot q starts out free for producer position q.
 __global__ void td16_queue_init_kernel(uint64_t *seq, uint64_t capPerShard, uint64_t total) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < total) seq[i] = i % capPerShard;
 }
 
-int launch_search(const Params &p, int blocks, void *stream) {
-  hipLaunchKernelGGL(td16_search_kernel, dim3(blocks), dim3(kBlockThreads), 0,
-                     (hipStream_t)stream, p);
+int launch_search(const Params &p, int blocks, bool wide, void *stream) {
+  if (wide)
+    hipLaunchKernelGGL(td16_search_kernel_w5, dim3(blocks), dim3(kBlockThreads), 0,
+                       (hipStream_t)stream, p);
+  else
+    hipLaunchKernelGGL(td16_search_kernel, dim3(blocks), dim3(kBlockThreads), 0,
+                       (hipStream_t)stream, p);
   return (int)hipGetLastError();
 }
 
@@ -1217,8 +1239,12 @@ int launch_queue_init(uint64_t *seq, uint64_t capPerShard, void *stream) {
   return (int)hipGetLastError();
 }
 
-int search_kernel_occupancy(int *blocksPerCU) {
-  return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCU, td16_search_kernel,
+int search_kernel_occupancy(int *blocksPerCU, int *blocksPerCUWide) {
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocksPerCU, td16_search_kernel, kBlockThreads, 0);
+  if (e != hipSuccess) return (int)e;
+  return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCUWide,
+                                                           td16_search_kernel_w5,
                                                            kBlockThreads, 0);
 }
 

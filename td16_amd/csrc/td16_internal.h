@@ -193,9 +193,10 @@ struct Params {
 };
 
 // Launch wrappers (td16_device.hip).
-int launch_search(const Params &p, int blocks, void *stream);
+// wide: the five-waves-per-SIMD build of the kernel (above 16 waves per CU)
+int launch_search(const Params &p, int blocks, bool wide, void *stream);
 int launch_queue_init(uint64_t *seq, uint64_t capPerShard, void *stream);
-int search_kernel_occupancy(int *blocksPerCU);
+int search_kernel_occupancy(int *blocksPerCU, int *blocksPerCUWide);
 
 // Host helpers (td16_host.cpp).
 int set_msg(char *msg, int cap, int status, const char *fmt, ...);

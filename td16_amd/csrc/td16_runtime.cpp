@@ -47,7 +47,8 @@ namespace {
 struct DeviceInfo {
   int device = -1;
   int cus = 0;
-  int blocksPerCU = 0;
+  int blocksPerCU = 0;      // resident blocks of the 4-waves-per-SIMD kernel
+  int blocksPerCUWide = 0;  // of the 5-waves-per-SIMD build
   uint16_t *dLex = nullptr;
 };
 
@@ -70,7 +71,7 @@ int device_info(int device, DeviceInfo *out, char *msg, int cap) {
   DeviceInfo d;
   d.device = device;
   TD16_HIP(hipDeviceGetAttribute(&d.cus, hipDeviceAttributeMultiprocessorCount, device));
-  TD16_HIP((hipError_t)search_kernel_occupancy(&d.blocksPerCU));
+  TD16_HIP((hipError_t)search_kernel_occupancy(&d.blocksPerCU, &d.blocksPerCUWide));
   if (d.blocksPerCU < 1) d.blocksPerCU = 1;
   std::vector<uint16_t> lex(kLexEntries);
   build_lex_table(lex.data());
@@ -263,7 +264,7 @@ int run_plain(Slot &s, const td16_request *req, td16_result *res, char *msg, int
   // they poll every ~1.6 us instead of the batched ~13 us (measured: -5 us
   // per call, +8 % single-slot and +14 % seven-slot request rate)
   P.idleBackoff = 4;
-  TD16_HIP((hipError_t)launch_search(P, s.blocks, s.stream));
+  TD16_HIP((hipError_t)launch_search(P, s.blocks, false, s.stream));
   TD16_HIP(hipEventRecord(s.done, s.stream));
   TD16_HIP(hipEventSynchronize(s.done));
   decode(s.buf.hResults[0], res);
@@ -691,9 +692,13 @@ int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
   }
   Buffers &b = gBatch.buf;
   // 0 = the measured best for the batched mix: 16 waves per CU (4 per SIMD,
-  // the most the kernel's 128-VGPR budget allows)
+  // the most the 128-VGPR build allows)
   const int wantWaves = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
-  const int perCU = std::max(1, std::min(dev.blocksPerCU, wantWaves / kWavesPerBlock));
+  // above the 4-waves-per-SIMD kernel's residency (16 waves per CU) the
+  // 5-waves-per-SIMD build runs
+  const int wantBlocks = std::max(1, wantWaves / kWavesPerBlock);
+  const bool wide = wantBlocks > dev.blocksPerCU && dev.blocksPerCUWide > dev.blocksPerCU;
+  const int perCU = std::min(wantBlocks, wide ? dev.blocksPerCUWide : dev.blocksPerCU);
   const int blocks = dev.cus * perCU;
   localStats.blocks = blocks;
   localStats.waves = blocks * kWavesPerBlock;
@@ -715,7 +720,7 @@ int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
     TD16_HIP(hipStreamBeginCapture(gBatch.stream, hipStreamCaptureModeThreadLocal));
     // the graph holds the kernel alone; the copies and the events that time
     // the kernel stay outside it
-    launch_search(P, blocks, gBatch.stream);
+    launch_search(P, blocks, wide, gBatch.stream);
     TD16_HIP(hipStreamEndCapture(gBatch.stream, &g));
     hipError_t e = hipGraphInstantiate(&gBatch.graph, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -733,7 +738,7 @@ int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
     if (o.use_graph)
       TD16_HIP(hipGraphLaunch(gBatch.graph, gBatch.stream));
     else
-      TD16_HIP((hipError_t)launch_search(P, blocks, gBatch.stream));
+      TD16_HIP((hipError_t)launch_search(P, blocks, wide, gBatch.stream));
     TD16_HIP(hipEventRecord(gBatch.ev1, gBatch.stream));
     TD16_HIP(hipStreamSynchronize(gBatch.stream));
     const auto w1 = std::chrono::steady_clock::now();
@@ -846,7 +851,7 @@ int td16_autotune(const td16_request *reqs, int count, td16_result *results,
     return TD16_OK;
   };
 
-  static const int kWaves[] = {4, 8, 12, 16};
+  static const int kWaves[] = {4, 8, 12, 16, 20};
   static const int kSplit[] = {2, 3, 4, 5, 6, 7};
   static const int kDonate[] = {4, 8, 16, 32, 64};
   int bw = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
@@ -860,7 +865,7 @@ int td16_autotune(const td16_request *reqs, int count, td16_result *results,
     bool moved = false;
     for (int knob = 0; knob < 3; ++knob) {
       const int *vals = knob == 0 ? kWaves : knob == 1 ? kSplit : kDonate;
-      const int nv = knob == 0 ? 4 : knob == 1 ? 6 : 5;
+      const int nv = knob == 0 ? 5 : knob == 1 ? 6 : 5;
       for (int v = 0; v < nv; ++v) {
         const int w = knob == 0 ? vals[v] : bw;
         const int sd = knob == 1 ? vals[v] : bsd;
@@ -965,7 +970,7 @@ int td16_serve_start(const td16_serve_options *opt, char *msg, int cap) {
   // 0 = the measured best for latency: 4 waves per CU (one per SIMD)
   const int wantWaves = opt->waves_per_cu > 0 ? opt->waves_per_cu : 4;
   const int perCU = std::max(1, std::min(dev.blocksPerCU, wantWaves / kWavesPerBlock));
-  TD16_HIP((hipError_t)launch_search(P, dev.cus * perCU, S->stream));
+  TD16_HIP((hipError_t)launch_search(P, dev.cus * perCU, false, S->stream));
   S->running = true;
   Server *raw = S.release();
   raw->poller = std::thread(poll_loop, raw);

@@ -349,3 +349,13 @@ def test_autotune_rejects_bad_arguments_before_any_launch(native):
         native.autotune([req], repeats=0)
     with pytest.raises(Td16Error, match="repeats"):
         native.autotune([req], repeats=101)
+
+
+def test_bench_strided_sample_copies_every_kth_request(native):
+    import bench
+    ns, ks, perms = native.gen_products(1, 12, 16, 4, 9, 23)
+    reqs = bench.build_requests(native, ns, ks, perms)
+    sample = bench.strided_sample(reqs, 5)
+    assert len(sample) == 5
+    for i, q in enumerate(range(0, 23, 5)):
+        assert bytes(sample[i]) == bytes(reqs[q])

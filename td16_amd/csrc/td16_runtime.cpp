@@ -184,7 +184,9 @@ Params make_params(const Buffers &b, const DeviceInfo &dev) {
   // neutral on the per-request and served paths
   P.donateMax = 16;
   P.served = 0;
-  P.idleBackoff = 32;
+  // idle sleeps of up to ~6.4 us: 1 % faster on the batched mix than ~13 us
+  // (the launch ends sooner after its last request), neutral on long batches
+  P.idleBackoff = 16;
   return P;
 }
 

@@ -296,9 +296,9 @@ def main() -> int:
     ap.add_argument("--emitter-instances", type=int, default=1000000)
     ap.add_argument("--skip", default="",
                     help="comma list of: tune,plain,served,cpu,probe,cert,emitter,stage")
-    ap.add_argument("--knobs", default="", metavar="W,S,D",
-                    help="waves per CU, split depth, donate max for the batched launch "
-                         "instead of tuning them")
+    ap.add_argument("--knobs", default="", metavar="W,S,D,C",
+                    help="waves per CU, split depth, donate max, check interval for the "
+                         "batched launch instead of tuning them")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (twice) for counter "
                          "calibration and exit")
@@ -325,8 +325,11 @@ def main() -> int:
     knobs = {}
     tuning = None
     if args.knobs:
-        w, sd, dm = (int(x) for x in args.knobs.split(","))
-        knobs = dict(waves_per_cu=w, split_depth=sd, donate_max=dm)
+        vals = [int(x) for x in args.knobs.split(",")]
+        names = ("waves_per_cu", "split_depth", "donate_max", "check_every")
+        if not 3 <= len(vals) <= 4:
+            ap.error("--knobs takes W,S,D or W,S,D,C")
+        knobs = dict(zip(names, vals))
     elif "tune" not in skip:
         t0 = time.perf_counter()
         tuning = native.autotune(reqs, repeats=3, use_graph=use_graph)
@@ -419,7 +422,8 @@ def main() -> int:
                         ("hipGraph-captured " if use_graph else ""),
             "instances": len(ns),
             "waves": st.waves,
-            "knobs": knobs or "library defaults (16 waves/CU, split depth 5, donate 16)",
+            "knobs": knobs or "library defaults (16 waves/CU, split depth 5, donate 16, "
+                              "check every 16)",
         },
         "nodes_per_s": round(nodes_per_s, 1),
         "kernel_ms": round(st.kernel_ms, 4),

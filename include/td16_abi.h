@@ -175,6 +175,9 @@ typedef struct {
     int32_t use_graph;       /* 1 = launch through a captured hipGraph */
     int32_t repeats;         /* run the same batch this many times (>= 1) */
     int64_t queue_bytes;     /* HBM for the shared work queue; 0 = 256 MiB */
+    int32_t check_every;     /* node expansions between an item's sharing
+                                checks (at most 1024); 0 = default (16) */
+    int32_t reserved;        /* 0 */
 } td16_batch_options;
 
 typedef struct {
@@ -198,21 +201,25 @@ typedef struct {
     int32_t waves_per_cu;
     int32_t split_depth;
     int32_t donate_max;
+    int32_t check_every;
     int32_t waves;           /* wavefronts in the grid it gave */
+    int32_t reserved;
     double kernel_ms;        /* median device time of one launch */
 } td16_tune_point;
 
 /* Autotuner for the batched launch: picks waves_per_cu (wavefronts per CU),
  * split_depth (how far above the move bound a DFS frame may still be
- * donated: the stack depth at which work is shared) and donate_max (the
- * steal granularity) for this batch by timing it.  Coordinate descent from
- * `base` over waves_per_cu {4, 8, 12, 16, 20}, split_depth {2..7} and
- * donate_max {4, 8, 16, 32, 64}; each point is one untimed launch and
+ * donated: the stack depth at which work is shared), donate_max (the steal
+ * granularity) and check_every (how often a search looks for idle
+ * wavefronts to share with) for this batch by timing it.  Coordinate
+ * descent from `base` over waves_per_cu {4, 8, 12, 16, 20}, split_depth
+ * {2..7}, donate_max {4, 8, 16, 32, 64} and check_every {8, 16, 32}; each
+ * point is one untimed launch and
  * `repeats` timed ones (median), and a point must beat the current choice by
  * more than 1 % to replace it.  The knobs only change how work is shared:
  * every point's answers (found; the distance of iterative-deepening
  * requests; the counts of COUNT_ALL requests) must equal the first point's,
- * else TD16_DEVICE_ERROR.  On success `best` is `base` with the three knobs
+ * else TD16_DEVICE_ERROR.  On success `best` is `base` with the four knobs
  * set, results[q] holds request q's result from the first point and
  * points[0 .. min(*npoints, points_cap)) the timed configurations in the
  * order tried.  The reference picks none of these (its queue count comes

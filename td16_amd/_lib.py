@@ -95,6 +95,8 @@ class BatchOptions(ctypes.Structure):
         ("use_graph", ctypes.c_int32),
         ("repeats", ctypes.c_int32),
         ("queue_bytes", ctypes.c_int64),
+        ("check_every", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -142,7 +144,9 @@ class TunePoint(ctypes.Structure):
         ("waves_per_cu", ctypes.c_int32),
         ("split_depth", ctypes.c_int32),
         ("donate_max", ctypes.c_int32),
+        ("check_every", ctypes.c_int32),
         ("waves", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("kernel_ms", ctypes.c_double),
     ]
 

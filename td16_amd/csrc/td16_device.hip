@@ -760,11 +760,13 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
       w.path[d] = (uint16_t)mv;
     }
     wave_sync();
-    // sharing and cancellation checks.  Subtrees are donated only when
-    // wavefronts sit idle, no unstarted request is waiting for them and this
-    // item has proved heavy: while requests remain, request-level
-    // parallelism keeps every wavefront busy for free.
-    if (++c.sinceCheck >= 32) {
+    // sharing and cancellation checks, every P.checkEvery expansions (one
+    // device-scope round trip each; fewer expansions between them spread a
+    // heavy request sooner).  Subtrees are donated only when wavefronts sit
+    // idle, no unstarted request is waiting for them and this item has
+    // proved heavy: while requests remain, request-level parallelism keeps
+    // every wavefront busy for free.
+    if (++c.sinceCheck >= P.checkEvery) {
       c.sinceCheck = 0;
       // an item that has proved heavy gets the SIMD's issue priority over
       // the light requests sharing it, so it does not become the launch's

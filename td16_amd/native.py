@@ -122,12 +122,12 @@ class BatchRun:
 
 def batch(reqs: Sequence[Request], device: int = 0, waves_per_cu: int = 0, split_depth: int = 0,
           donate_max: int = 0, use_graph: bool = False, repeats: int = 1,
-          queue_bytes: int = 0) -> BatchRun:
+          queue_bytes: int = 0, check_every: int = 0) -> BatchRun:
     n = len(reqs)
     arr = (Request * max(1, n))(*reqs) if not isinstance(reqs, ctypes.Array) else reqs
     out = (Result * max(1, n))()
     opt = BatchOptions(device, waves_per_cu, split_depth, donate_max, 1 if use_graph else 0,
-                       repeats, queue_bytes)
+                       repeats, queue_bytes, check_every, 0)
     st = BatchStats()
     m = msgbuf()
     check(lib().td16_batch(arr, n, out, ctypes.byref(opt), ctypes.byref(st), m, len(m)), m)
@@ -136,14 +136,14 @@ def batch(reqs: Sequence[Request], device: int = 0, waves_per_cu: int = 0, split
 
 @dataclass
 class Tuning:
-    best: dict            # waves_per_cu, split_depth, donate_max
+    best: dict            # waves_per_cu, split_depth, donate_max, check_every
     points: List[dict]    # every timed configuration, in the order tried
     results: ctypes.Array
 
 
 def autotune(reqs: Sequence[Request], device: int = 0, repeats: int = 3, use_graph: bool = True,
              queue_bytes: int = 0, waves_per_cu: int = 0, split_depth: int = 0,
-             donate_max: int = 0, max_points: int = 64) -> Tuning:
+             donate_max: int = 0, check_every: int = 0, max_points: int = 64) -> Tuning:
     """Times the batch over the knob grid of ``td16_autotune`` and returns the
     fastest setting; pass ``best`` to :func:`batch`.  Raises ``Td16Error`` if
     any setting changed an answer."""
@@ -151,7 +151,7 @@ def autotune(reqs: Sequence[Request], device: int = 0, repeats: int = 3, use_gra
     arr = (Request * max(1, n))(*reqs) if not isinstance(reqs, ctypes.Array) else reqs
     out = (Result * max(1, n))()
     base = BatchOptions(device, waves_per_cu, split_depth, donate_max, 1 if use_graph else 0, 1,
-                        queue_bytes)
+                        queue_bytes, check_every, 0)
     best = BatchOptions()
     pts = (TunePoint * max_points)()
     npts = ctypes.c_int32(0)
@@ -159,10 +159,11 @@ def autotune(reqs: Sequence[Request], device: int = 0, repeats: int = 3, use_gra
     check(lib().td16_autotune(arr, n, out, ctypes.byref(base), repeats, ctypes.byref(best), pts,
                               max_points, ctypes.byref(npts), m, len(m)), m)
     points = [dict(waves_per_cu=p.waves_per_cu, split_depth=p.split_depth,
-                   donate_max=p.donate_max, waves=p.waves, kernel_ms=p.kernel_ms)
+                   donate_max=p.donate_max, check_every=p.check_every, waves=p.waves,
+                   kernel_ms=p.kernel_ms)
               for p in pts[:min(npts.value, max_points)]]
     return Tuning(dict(waves_per_cu=best.waves_per_cu, split_depth=best.split_depth,
-                       donate_max=best.donate_max), points, out)
+                       donate_max=best.donate_max, check_every=best.check_every), points, out)
 
 
 # ---------------------------------------------------------------------------

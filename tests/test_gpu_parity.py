@@ -226,13 +226,13 @@ def test_unoriented_stage_partitions_all_solved(O, native):
 
 # Settings of the three sharing knobs (wavefronts per CU, donation depth,
 # steal granularity), from the sparsest grid to the densest.
-KNOB_POINTS = [dict(waves_per_cu=4, split_depth=2, donate_max=4),
-               dict(waves_per_cu=8, split_depth=7, donate_max=64),
-               dict(waves_per_cu=16, split_depth=3, donate_max=16),
-               dict(waves_per_cu=12, split_depth=5, donate_max=32),
+KNOB_POINTS = [dict(waves_per_cu=4, split_depth=2, donate_max=4, check_every=1),
+               dict(waves_per_cu=8, split_depth=7, donate_max=64, check_every=32),
+               dict(waves_per_cu=16, split_depth=3, donate_max=16, check_every=8),
+               dict(waves_per_cu=12, split_depth=5, donate_max=32, check_every=1024),
                # the 5-waves-per-SIMD build of the kernel
-               dict(waves_per_cu=20, split_depth=4, donate_max=8)]
-KNOBS = ("waves_per_cu", "split_depth", "donate_max")
+               dict(waves_per_cu=20, split_depth=4, donate_max=8, check_every=16)]
+KNOBS = ("waves_per_cu", "split_depth", "donate_max", "check_every")
 
 
 def knob_requests(native):
@@ -260,9 +260,9 @@ def test_sharing_knobs_never_change_answers(O, native):
 def test_autotune_picks_a_timed_point_and_keeps_answers(O, native):
     reqs = knob_requests(native)
     t = native.autotune(reqs, repeats=1)
-    assert tuple(t.points[0][k] for k in KNOBS) == (16, 5, 16)  # the library defaults
+    assert tuple(t.points[0][k] for k in KNOBS) == (16, 5, 16, 16)  # the library defaults
     # the first pass times every value of every knob once
-    assert len(t.points) >= 5 + 5 + 4
+    assert len(t.points) >= 5 + 5 + 4 + 2
     chosen = [p for p in t.points if all(p[k] == t.best[k] for k in KNOBS)]
     assert len(chosen) == 1
     assert chosen[0]["kernel_ms"] <= t.points[0]["kernel_ms"]

@@ -201,3 +201,28 @@ def test_dispatcher_smoke_like_reference():
             WitnessSearch.uninstall_writer()
     finally:
         GPUSortingSearch.reset()
+
+
+def test_config1_full_size_properties(O, native):
+    """All 10,000 uniform n = 10 permutations (seed 0), exact distance by
+    iterative deepening in both move sets, checked by what holds at any size:
+    every witness replays to the identity, the distance lies between the
+    odd-cycle bound (N - odd) / 2 and the diameter d(10) = 6, and the
+    reference filter never beats the full move set."""
+    perms = native.gen_uniform(0, 10, 10000)
+    depth = {}
+    for ms in (native.MOVES_ALL, native.MOVES_REF):
+        reqs = [native.linear_request(int(p), 10, 9, ms, native.FIRST, True) for p in perms]
+        run = native.batch(reqs)
+        d = []
+        for q, req in enumerate(reqs):
+            r = run.results[q]
+            assert r.found == 1, q
+            lb = (req.n - req.initial_even) // 2
+            assert lb <= r.depth <= 6, (q, lb, r.depth)
+            beta, omega = pair(req)
+            sym = native.positions_to_symbols(beta, r.witness())
+            assert O.validate(beta, omega, sym, req.max_moves) == 0, q
+            d.append(r.depth)
+        depth[ms] = d
+    assert all(a <= b for a, b in zip(depth[native.MOVES_ALL], depth[native.MOVES_REF]))

@@ -1094,11 +1094,11 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
         });
       }
       if (done) break;
-      // idle polling touches a few words of HBM; back off to keep those
-      // cache lines cool (about 0.4 us per unit, up to ~3.5 us served and
-      // P.idleBackoff units batched)
+      // idle polling touches a few words of HBM and contends with the
+      // working wavefronts' atomics and cache write-backs; back off (about
+      // 0.4 us per unit, up to P.idleBackoff units)
       for (int q = 0; q < backoff; ++q) __builtin_amdgcn_s_sleep(16);
-      const int maxBackoff = P.served ? 8 : P.idleBackoff;
+      const int maxBackoff = P.idleBackoff;
       backoff = backoff < maxBackoff ? backoff * 2 : maxBackoff;
       continue;
     }

@@ -184,7 +184,7 @@ struct Params {
   int32_t splitSlack;    // donate only frames with depth <= limit - splitSlack
   int32_t donateMax;
   int32_t served;        // 0 batched, 1 served loop
-  int32_t idleBackoff;   // batched: longest idle sleep, in units of s_sleep(16)
+  int32_t idleBackoff;   // longest idle sleep, in units of s_sleep(16)
   int32_t checkEvery;    // expansions between an item's sharing checks
   uint32_t ringCap;      // served: power of two
   ServeHost *host;       // served: device view of the host block

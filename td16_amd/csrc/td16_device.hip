@@ -229,7 +229,6 @@ enum { kExpandOk = 0, kExpandFound = 1 };
 // (tens of thousands of nodes on average) at the first check.
 constexpr int kHeavyNodes = 64;
 constexpr int kHeavyNodesDeep = 0;
-constexpr int kDeepBound = 8;
 constexpr int kHeavyNodesSlack = 512;
 constexpr int kEarlyStock = 1024;
 

@@ -34,6 +34,12 @@ enum JobFlags : uint8_t {
                        // depth at which a witness is possible (a long search)
 };
 
+// Requests without slack and with a move bound of at least kDeepBound are
+// the long searches (tens of thousands of nodes on average on the batched
+// mix): they are split at the first check, and a per-request launch gives
+// them the slot's whole grid.
+constexpr int kDeepBound = 8;
+
 // One request as the kernel reads it (host-written, 128 B).
 struct alignas(16) Job {
   uint64_t pi0;          // start sequence (labels), nibble-packed

@@ -271,7 +271,15 @@ int run_plain(Slot &s, const td16_request *req, td16_result *res, char *msg, int
   // they poll every ~1.6 us instead of the batched ~13 us (measured: -5 us
   // per call, +8 % single-slot and +14 % seven-slot request rate)
   P.idleBackoff = 4;
-  TD16_HIP((hipError_t)launch_search(P, s.blocks, false, s.stream));
+  // a request that may run long gets the slot's whole grid; the rest run on
+  // 32 blocks (128 wavefronts), whose idle pollers disturb its one working
+  // wavefront far less than the slot's ~1,000 blocks would (measured on the
+  // batched mix's requests: single-slot calls +40 %, p50 54 -> 37 us; its
+  // heaviest requests, up to 5,271 nodes, take as long on 16 blocks as on
+  // the whole chip)
+  const bool longSearch = (job.flags & kFlagTight) && job.maxDepth >= kDeepBound;
+  const int blocks = longSearch ? s.blocks : std::min(s.blocks, 32);
+  TD16_HIP((hipError_t)launch_search(P, blocks, false, s.stream));
   TD16_HIP(hipEventRecord(s.done, s.stream));
   TD16_HIP(hipEventSynchronize(s.done));
   decode(s.buf.hResults[0], res);

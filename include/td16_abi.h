@@ -196,6 +196,33 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
                const td16_batch_options *opt, td16_batch_stats *stats,
                char *msg, int msg_cap);
 
+/* Allocation-free variant of td16_batch.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * running while a HIP graph is captured, tight serving loops, harnesses that
+ * forbid hipMalloc.  It takes the same requests, results, options and
+ * statistics as td16_batch and runs the same kernels, but calls no
+ * hipMalloc, hipFree, hipHostMalloc or other allocating HIP routine: every
+ * device buffer is a region of one static device array per GPU of at most
+ * TD16_NOALLOC_SCRATCH_BYTES, which that GPU holds until the process ends.
+ * Host staging uses ordinary (pageable) process memory.  Differences:
+ *  - count must be at most TD16_NOALLOC_MAX_REQUESTS, else
+ *    TD16_INVALID_ARGUMENT;
+ *  - the shared work queue holds TD16_NOALLOC_QUEUE_ITEMS donated subtrees
+ *    (queue_bytes is ignored; a full queue only stops donation, so results
+ *    do not depend on it);
+ *  - use_graph must be 0; launches go to the null stream;
+ *  - kernel_ms is host-timed (launch to completion, no events).
+ * Every call on one GPU uses that GPU's array, which keeps the last call's
+ * contents (each call resets what a launch reads before writing).  Calls
+ * that overlapped on one GPU would race on it, so the library serializes
+ * them per process with a lock; each process has its own arrays. */
+#define TD16_NOALLOC_MAX_REQUESTS 32768
+#define TD16_NOALLOC_QUEUE_ITEMS 32768
+#define TD16_NOALLOC_SCRATCH_BYTES (13 << 20)
+int td16_batch_noalloc(const td16_request *reqs, int count, td16_result *results,
+                       const td16_batch_options *opt, td16_batch_stats *stats,
+                       char *msg, int msg_cap);
+
 /* One configuration the autotuner timed. */
 typedef struct {
     int32_t waves_per_cu;

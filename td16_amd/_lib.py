@@ -17,6 +17,8 @@ MAX_DEPTH = 9
 CERT_KEY_BYTES = 11
 CERT_VALUE_BYTES = 17
 BRACELET_PAIRS, BRACELET_WORDS = 0, 1
+NOALLOC_MAX_REQUESTS = 32768
+NOALLOC_QUEUE_ITEMS = 32768
 
 OK = 0
 INVALID_ARGUMENT = 1
@@ -162,6 +164,8 @@ SIGNATURES = {
     "td16_search_ex": (_i, [_i, ctypes.POINTER(Request), ctypes.POINTER(Result), _c, _i]),
     "td16_batch": (_i, [ctypes.POINTER(Request), _i, ctypes.POINTER(Result),
                         ctypes.POINTER(BatchOptions), ctypes.POINTER(BatchStats), _c, _i]),
+    "td16_batch_noalloc": (_i, [ctypes.POINTER(Request), _i, ctypes.POINTER(Result),
+                                ctypes.POINTER(BatchOptions), ctypes.POINTER(BatchStats), _c, _i]),
     "td16_autotune": (_i, [ctypes.POINTER(Request), _i, ctypes.POINTER(Result),
                            ctypes.POINTER(BatchOptions), _i, ctypes.POINTER(BatchOptions),
                            ctypes.POINTER(TunePoint), _i, _i32p, _c, _i]),

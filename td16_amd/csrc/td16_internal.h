@@ -15,6 +15,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
+
+#include "../../include/td16_abi.h"
 
 namespace td16 {
 
@@ -211,5 +214,21 @@ void build_lex_table(uint16_t *out);   // kLexEntries entries
 int rate_tables(int N, int initialEven, float minRate, int maxDepth, int fullSorting,
                 uint32_t sat[10], uint32_t reach[10]);
 void unpack_moves(const uint16_t *packed, int depth, int32_t (*moves)[3]);
+
+// Launch-path helpers shared by the batched entry points (td16_runtime.cpp).
+// The library's default sharing knobs, and a batch's overrides of them.
+void default_knobs(Params &P);
+void apply_batch_knobs(Params &P, const td16_batch_options &o);
+// Persistent grid for `wavesPerCU` (0 = default) on a device with `cus` CUs:
+// blocks, and whether the five-waves-per-SIMD build runs.
+void batch_grid(int wavesPerCU, int cus, int blocksPerCU, int blocksPerCUWide, int *blocks,
+                bool *wide);
+void decode_result(const DevResult &r, td16_result *out);
+void trivial_result(int trivial, td16_result *out);
+// Validates and relabels every request, answers the trivial ones in
+// `results` and returns the rest in feed order (`where[q]` is the caller's
+// index of jobs[q]).
+int prepare_batch(const td16_request *reqs, int count, td16_result *results,
+                  std::vector<Job> &jobs, std::vector<int> &where, char *msg, int cap);
 
 }  // namespace td16

@@ -164,20 +164,7 @@ size_t init_feed(Control *c, uint32_t numJobs) {
   return offsetof(Control, feed) + (size_t)used * sizeof(FeedShard);
 }
 
-namespace {
-
-Params make_params(const Buffers &b, const DeviceInfo &dev) {
-  Params P;
-  std::memset(&P, 0, sizeof(P));
-  P.jobs = b.dJobs;
-  P.states = b.dStates;
-  P.results = b.dResults;
-  P.ctrl = b.dCtrl;
-  P.queue = b.dQueue;
-  P.items = b.dItems;
-  P.seq = b.dSeq;
-  P.qcap = b.qcap;
-  P.lex = dev.dLex;
+void default_knobs(Params &P) {
   P.splitSlack = 5;
   // children per donation: the autotuner's pick on the batched mix (0.80
   // against 0.84 ms per launch at 64), 1 % faster on the 1M emitter stream,
@@ -192,10 +179,28 @@ Params make_params(const Buffers &b, const DeviceInfo &dev) {
   // heaviest request alone 0.15 against 0.25 ms, the 1M emitter stream
   // 48.6 against 51.5 ms); at 8 the checks cost the mix more than they gain
   P.checkEvery = 16;
-  return P;
 }
 
-void decode(const DevResult &r, td16_result *out) {
+void apply_batch_knobs(Params &P, const td16_batch_options &o) {
+  if (o.split_depth > 0) P.splitSlack = o.split_depth;
+  if (o.donate_max > 0) P.donateMax = std::min(o.donate_max, 64);
+  if (o.check_every > 0) P.checkEvery = std::min(o.check_every, 1024);
+}
+
+void batch_grid(int wavesPerCU, int cus, int blocksPerCU, int blocksPerCUWide, int *blocks,
+                bool *wide) {
+  // 0 = the measured best for the batched mix: 16 waves per CU (4 per SIMD,
+  // the most the 128-VGPR build allows)
+  const int wantWaves = wavesPerCU > 0 ? wavesPerCU : 16;
+  // above the 4-waves-per-SIMD kernel's residency (16 waves per CU) the
+  // 5-waves-per-SIMD build runs
+  const int wantBlocks = std::max(1, wantWaves / kWavesPerBlock);
+  *wide = wantBlocks > blocksPerCU && blocksPerCUWide > blocksPerCU;
+  const int perCU = std::min(wantBlocks, *wide ? blocksPerCUWide : blocksPerCU);
+  *blocks = cus * std::max(1, perCU);
+}
+
+void decode_result(const DevResult &r, td16_result *out) {
   std::memset(out, 0, sizeof(*out));
   out->status = TD16_OK;
   out->found = r.found;
@@ -211,6 +216,24 @@ void trivial_result(int trivial, td16_result *out) {
   std::memset(out, 0, sizeof(*out));
   out->status = TD16_OK;
   if (trivial == 2) out->found = 1;  // IDDFS on a sorted input: distance 0
+}
+
+namespace {
+
+Params make_params(const Buffers &b, const DeviceInfo &dev) {
+  Params P;
+  std::memset(&P, 0, sizeof(P));
+  P.jobs = b.dJobs;
+  P.states = b.dStates;
+  P.results = b.dResults;
+  P.ctrl = b.dCtrl;
+  P.queue = b.dQueue;
+  P.items = b.dItems;
+  P.seq = b.dSeq;
+  P.qcap = b.qcap;
+  P.lex = dev.dLex;
+  default_knobs(P);
+  return P;
 }
 
 // ---------------------------------------------------------------------------
@@ -282,7 +305,7 @@ int run_plain(Slot &s, const td16_request *req, td16_result *res, char *msg, int
   TD16_HIP((hipError_t)launch_search(P, blocks, false, s.stream));
   TD16_HIP(hipEventRecord(s.done, s.stream));
   TD16_HIP(hipEventSynchronize(s.done));
-  decode(s.buf.hResults[0], res);
+  decode_result(s.buf.hResults[0], res);
   return TD16_OK;
 }
 
@@ -414,7 +437,7 @@ void poll_loop(Server *S) {
       const uint64_t t = S->slotTicket[slot];
       auto it = S->tickets.find(t);
       if (it != S->tickets.end()) {
-        decode(r, &it->second.res);
+        decode_result(r, &it->second.res);
         it->second.ready = true;
         const double us = std::chrono::duration<double, std::micro>(now - it->second.t0).count();
         it->second.res.latency_us = us;
@@ -602,6 +625,8 @@ td16_batch_options normalize_batch_options(const td16_batch_options *opt) {
   return o;
 }
 
+}  // namespace
+
 // Host side of a batch: validates and relabels every request, answers the
 // trivial ones in `results`, and returns the rest in feed order (`where[q]`
 // is the caller's index of jobs[q]).
@@ -669,6 +694,8 @@ int prepare_batch(const td16_request *reqs, int count, td16_result *results,
   return TD16_OK;
 }
 
+namespace {
+
 // Device side of a batch: `o.repeats` persistent launches over the prepared
 // requests; results of the last one land in results[where[q]].
 int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
@@ -706,15 +733,9 @@ int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
     if (st != TD16_OK) return st;
   }
   Buffers &b = gBatch.buf;
-  // 0 = the measured best for the batched mix: 16 waves per CU (4 per SIMD,
-  // the most the 128-VGPR build allows)
-  const int wantWaves = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
-  // above the 4-waves-per-SIMD kernel's residency (16 waves per CU) the
-  // 5-waves-per-SIMD build runs
-  const int wantBlocks = std::max(1, wantWaves / kWavesPerBlock);
-  const bool wide = wantBlocks > dev.blocksPerCU && dev.blocksPerCUWide > dev.blocksPerCU;
-  const int perCU = std::min(wantBlocks, wide ? dev.blocksPerCUWide : dev.blocksPerCU);
-  const int blocks = dev.cus * perCU;
+  int blocks = 0;
+  bool wide = false;
+  batch_grid(o.waves_per_cu, dev.cus, dev.blocksPerCU, dev.blocksPerCUWide, &blocks, &wide);
   localStats.blocks = blocks;
   localStats.waves = blocks * kWavesPerBlock;
   if (nj == 0) {
@@ -726,9 +747,7 @@ int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
   std::memset(b.hCtrl, 0, sizeof(Control));
   const size_t ctrlBytes = init_feed(b.hCtrl, nj);
   Params P = make_params(b, dev);
-  if (o.split_depth > 0) P.splitSlack = o.split_depth;
-  if (o.donate_max > 0) P.donateMax = std::min(o.donate_max, 64);
-  if (o.check_every > 0) P.checkEvery = std::min(o.check_every, 1024);
+  apply_batch_knobs(P, o);
 
   if (o.use_graph) {
     reset_batch_graph();
@@ -768,7 +787,7 @@ int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
   for (uint32_t q = 0; q < nj; ++q) {
     const DevResult &r = b.hResults[q];
     td16_result &out = results[where[q]];
-    decode(r, &out);
+    decode_result(r, &out);
     localStats.expanded += r.expanded;
     localStats.solutions += r.solutions;
     localStats.donations += (uint64_t)r.split;

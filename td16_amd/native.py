@@ -134,6 +134,23 @@ def batch(reqs: Sequence[Request], device: int = 0, waves_per_cu: int = 0, split
     return BatchRun(out, st)
 
 
+def batch_noalloc(reqs: Sequence[Request], device: int = 0, waves_per_cu: int = 0,
+                  split_depth: int = 0, donate_max: int = 0, repeats: int = 1,
+                  check_every: int = 0, use_graph: bool = False) -> BatchRun:
+    """:func:`batch` through ``td16_batch_noalloc``: no device allocation, at
+    most ``NOALLOC_MAX_REQUESTS`` requests per call."""
+    n = len(reqs)
+    arr = (Request * max(1, n))(*reqs) if not isinstance(reqs, ctypes.Array) else reqs
+    out = (Result * max(1, n))()
+    opt = BatchOptions(device, waves_per_cu, split_depth, donate_max, 1 if use_graph else 0,
+                       repeats, 0, check_every, 0)
+    st = BatchStats()
+    m = msgbuf()
+    check(lib().td16_batch_noalloc(arr, n, out, ctypes.byref(opt), ctypes.byref(st), m, len(m)),
+          m)
+    return BatchRun(out, st)
+
+
 @dataclass
 class Tuning:
     best: dict            # waves_per_cu, split_depth, donate_max, check_every

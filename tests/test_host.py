@@ -359,3 +359,15 @@ def test_bench_strided_sample_copies_every_kth_request(native):
     assert len(sample) == 5
     for i, q in enumerate(range(0, 23, 5)):
         assert bytes(sample[i]) == bytes(reqs[q])
+
+
+def test_noalloc_rejects_oversized_batches_before_touching_a_gpu(native):
+    # the size and option checks come before any HIP call, so they hold here
+    reqs = (_lib.Request * (_lib.NOALLOC_MAX_REQUESTS + 1))()
+    with pytest.raises(Td16Error) as e:
+        native.batch_noalloc(reqs)
+    assert e.value.status == _lib.INVALID_ARGUMENT
+    assert "32768" in str(e.value) and "maximum" in str(e.value)
+    with pytest.raises(Td16Error) as e:
+        native.batch_noalloc(reqs[:4], use_graph=True)
+    assert e.value.status == _lib.INVALID_ARGUMENT and "use_graph" in str(e.value)

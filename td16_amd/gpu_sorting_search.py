@@ -14,13 +14,10 @@ from __future__ import annotations
 import os
 import queue
 import threading
-import time
 from concurrent.futures import Future
 from typing import List, Optional, Sequence, Tuple
 
 from . import native
-from .device_layout import GpuDeviceLayout
-from .search_stats import GpuSearchStats
 
 
 def _prop(name: str, default: str = "") -> str:
@@ -37,8 +34,14 @@ class GPUSortingSearch:
                  served_slots: int = 0):
         if queue_bytes_budget <= 0:
             raise ValueError("queueBytesBudget must be positive")
-        self.layout = GpuDeviceLayout(devices, slots_per_device)
-        self.total_slots = self.layout.total_slots()
+        self.devices = [int(d) for d in devices]
+        if not self.devices or min(self.devices) < 0 or \
+                len(set(self.devices)) != len(self.devices):
+            raise ValueError("the device list needs distinct non-negative ordinals, got %r"
+                             % (list(devices),))
+        if slots_per_device < 1:
+            raise ValueError("slots per device must be at least 1, got %d" % slots_per_device)
+        self.total_slots = len(self.devices) * slots_per_device
         if queue_capacity is None:
             v = _prop("GPU_QUEUE_CAPACITY")
             queue_capacity = int(v) if v else 0
@@ -47,21 +50,26 @@ class GPUSortingSearch:
         self.queue_capacity = queue_capacity
         self.pending_capacity = self.total_slots + queue_capacity
         self._pending: "queue.Queue" = queue.Queue(maxsize=self.pending_capacity)
-        self.stats = GpuSearchStats(self.layout, time.monotonic_ns())
+        # finished searches, successful and failed (the Java side keeps its
+        # own windowed statistics around the native call)
+        self._count_lock = threading.Lock()
+        self.completed = 0
+        self.failed = 0
         self.served = served
         if served:
-            if len(self.layout.devices()) != 1:
+            if len(self.devices) != 1:
                 raise ValueError("the served loop runs on one device")
-            native.serve_start(device=self.layout.devices()[0],
+            native.serve_start(device=self.devices[0],
                                slots=served_slots or max(self.total_slots, 64),
                                queue_capacity=max(queue_capacity, 1),
                                slot_bytes=queue_bytes_budget)
         else:
-            native.init_slots(self.total_slots, self.layout.devices(), queue_bytes_budget)
+            # slot s runs on devices[s % len(devices)] (td16_init_slots)
+            native.init_slots(self.total_slots, self.devices, queue_bytes_budget)
         self._threads = []
         for slot in range(self.total_slots):
             t = threading.Thread(target=self._run_slot, args=(slot,), daemon=True,
-                                 name=f"gpu-device-{self.layout.device_for_slot(slot)}-slot-{slot}")
+                                 name="td16-slot-%d" % slot)
             t.start()
             self._threads.append(t)
 
@@ -101,9 +109,8 @@ class GPUSortingSearch:
             job = self._pending.get()
             if job is None:
                 return
-            beta, omega, initial_even, min_rate, max_moves, full_sorting, queued_at, fut = job
+            beta, omega, initial_even, min_rate, max_moves, full_sorting, fut = job
             failed = True
-            self.stats.acquired(slot, queued_at, time.monotonic_ns())
             try:
                 if self.served:
                     req = native.make_request(beta, omega, initial_even, min_rate, max_moves,
@@ -118,7 +125,9 @@ class GPUSortingSearch:
             except BaseException as e:  # the job completes exceptionally (:150-152)
                 fut.set_exception(e)
             finally:
-                self.stats.released(slot, time.monotonic_ns(), failed)
+                with self._count_lock:
+                    self.completed += 0 if failed else 1
+                    self.failed += 1 if failed else 0
 
     # -- search (GPUSortingSearch.java:203-240) ----------------------------
     def search(self, beta: Sequence[int], omega: Sequence[int], initial_even: int,
@@ -126,7 +135,7 @@ class GPUSortingSearch:
                ) -> List[Tuple[int, int, int]]:
         fut: Future = Future()
         self._pending.put((list(beta), list(omega), initial_even, min_rate, max_moves,
-                           full_sorting, time.monotonic_ns(), fut))
+                           full_sorting, fut))
         positions = fut.result()
         if not positions:
             return []

@@ -8,7 +8,7 @@ import re
 import numpy as np
 import pytest
 
-from td16_amd import GpuDeviceLayout, GpuSearchStats, Td16Error
+from td16_amd import Td16Error
 from td16_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -109,6 +109,32 @@ def test_codec_matches_oracle(O, native):
         native.codec_value([])
     with pytest.raises(Td16Error):
         native.codec_value([(0, 1, 32)])
+
+
+def test_codec_bytes_written_out_by_hand(native):
+    """The certificate layout (CertificateCodec.java:22-70): each symbol in 5
+    bits, most significant bit first, concatenated and zero-padded to a
+    byte.  These bytes were worked out on paper from that definition, so a
+    mistake shared by the product and the oracle (bit order, padding side)
+    fails here."""
+    # [1, 2, 0] -> 00001 00010 00000 + 0 -> 0000 1000 | 1000 0000
+    assert native.codec_key([1, 2, 0]) == bytes([0x08, 0x80])
+    # (0, 1, 2) -> 00000 00001 00010 + 0 -> 0000 0000 | 0100 0100
+    assert native.codec_value([(0, 1, 2)]) == bytes([0x00, 0x44])
+    # 17 symbols, the reverse 16..0: 85 bits and 3 padding bits in 11 bytes
+    rev = list(range(16, -1, -1))
+    key = bytes.fromhex("83 dc d6 2d 49 41 cc 52 0c 41 00")
+    assert native.codec_key(rev) == key
+    assert native.codec_decode_key(key, 17) == rev
+    # two moves: 10000 00000 00101 00011 01001 01100 + 00
+    val = bytes.fromhex("80 0a 34 b0")
+    assert native.codec_value([(16, 0, 5), (3, 9, 12)]) == val
+    assert native.codec_decode_value(val) == [(16, 0, 5), (3, 9, 12)]
+    # a set padding bit is rejected by both decoders
+    with pytest.raises(Td16Error):
+        native.codec_decode_key(bytes([0x08, 0x81]), 3)
+    with pytest.raises(Td16Error):
+        native.codec_decode_value(bytes([0x00, 0x45]))
 
 
 # oracle replay codes -> td16_check_batch / td16_certify_batch status
@@ -280,64 +306,13 @@ def test_search_argument_errors_without_gpu(native):
     assert e.value.status == _lib.NOT_INITIALIZED
 
 
-# --- GpuDeviceLayout (ET/base/GpuDeviceLayoutTest.java:11-44) -------------
-
-def test_layout_parses_and_maps_round_robin():
-    devices = GpuDeviceLayout.parse_devices(" 2, 5 ")
-    layout = GpuDeviceLayout(devices, 3)
-    assert layout.devices() == [2, 5]
-    assert layout.total_slots() == 6
-    assert [layout.device_for_slot(s) for s in (0, 1, 2, 5)] == [2, 5, 2, 5]
-
-
-def test_layout_rejects_duplicate_or_malformed():
-    for bad in (lambda: GpuDeviceLayout([0, 0], 7), lambda: GpuDeviceLayout([-1], 7),
-                lambda: GpuDeviceLayout.parse_devices("0,,1"),
-                lambda: GpuDeviceLayout.parse_devices("gpu0"),
-                lambda: GpuDeviceLayout.parse_devices(" ")):
+def test_dispatcher_rejects_bad_device_lists_before_any_gpu_call():
+    from td16_amd import GPUSortingSearch
+    for devices, slots in (([0, 0], 7), ([-1], 7), ([], 7), ([0], 0)):
         with pytest.raises(ValueError):
-            bad()
-
-
-def test_layout_rejects_invalid_or_overflowing_slot_counts():
+            GPUSortingSearch(devices, slots, 1 << 20)
     with pytest.raises(ValueError):
-        GpuDeviceLayout([0], 0)
-    with pytest.raises(ValueError):
-        GpuDeviceLayout([0, 1], 2**31 - 1)
-    with pytest.raises(ValueError):
-        GpuDeviceLayout([0], 2).device_for_slot(2)
-
-
-# --- GpuSearchStats (ET/base/GpuSearchStatsTest.java:9-55) ----------------
-
-def test_stats_attribute_wait_busy_and_completions():
-    stats = GpuSearchStats(GpuDeviceLayout([0, 1], 2), 0)
-    stats.acquired(0, 5, 10)
-    stats.acquired(1, 10, 20)
-    stats.released(0, 60, False)
-    first = stats.snapshot(100)
-    assert (first.started, first.completed, first.failed) == (2, 1, 0)
-    assert first.queue_wait_nanos == 15 and first.busy_nanos == 130
-    assert first.devices[0].completed == 1 and first.devices[0].busy_nanos == 50
-    assert first.devices[1].completed == 0 and first.devices[1].busy_nanos == 80
-    stats.released(1, 180, True)
-    second = stats.snapshot(200)
-    assert (second.started, second.completed, second.failed) == (0, 1, 1)
-    assert second.busy_nanos == 80
-    assert second.devices[0].busy_nanos == 0 and second.devices[1].busy_nanos == 80
-
-
-def test_stats_window_reset_keeps_active_search():
-    stats = GpuSearchStats(GpuDeviceLayout([3], 1), 1000)
-    stats.acquired(0, 1000, 1100)
-    first = stats.snapshot(1500)
-    second = stats.snapshot(1800)
-    stats.released(0, 2000, False)
-    third = stats.snapshot(2100)
-    assert (first.busy_nanos, second.busy_nanos, third.busy_nanos) == (400, 300, 200)
-    assert third.completed == 1 and third.started == 0
-    with pytest.raises(RuntimeError):
-        stats.released(0, 2200, False)
+        GPUSortingSearch([0], 2, 0)
 
 
 def test_autotune_rejects_bad_arguments_before_any_launch(native):

@@ -111,13 +111,13 @@ def certificate_rates(native, reqs, results, repeats=5):
             "check_per_s": round(valid / best_k, 1)}
 
 
-def plain_rate(native, reqs, count, slots):
-    """Witnesses/s of the blocking one-request-per-launch path with `slots`
-    slots driven by one thread each (the reference's orchestration)."""
+def plain_run(native, reqs, count, slots):
+    """Requests/s of the blocking one-request-per-launch path with `slots`
+    slots driven by one thread each (the reference's orchestration); returns
+    (rate, seconds, results)."""
     native.init_slots(slots, [0], 1 << 30)
     try:
         out = [None] * count
-        # warm every slot once
         for s in range(slots):
             native.search_ex(s, reqs[s % count])
 
@@ -134,6 +134,13 @@ def plain_rate(native, reqs, count, slots):
         dt = time.perf_counter() - t0
     finally:
         native.shutdown()
+    return count / dt, dt, out
+
+
+def plain_rate(native, reqs, count, slots):
+    """Verified witnesses/s of the plain path (plain_run) on the first
+    `count` requests."""
+    _, dt, out = plain_run(native, reqs, count, slots)
     found, verified = verify(native, reqs[:count], out)
     return verified / dt, dt, found, verified
 
@@ -162,6 +169,175 @@ def cpu_baseline(native, ns, ks, perms, seconds_target=15.0):
         "sample": "first %d instances of the same mix, C restatement of the search in the "
                   "kernel's default (greedy-first) order (walk-based delta) + certificate "
                   "replay, %d threads, %.1f s; %.3g nodes/s" % (count, cores, dt, nodes / dt),
+    }
+
+
+GOLDEN = os.path.join(ROOT, "tests", "golden", "expected.json")
+
+
+def golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def concat_requests(arrays):
+    """One ctypes request array holding the given arrays back to back."""
+    from td16_amd._lib import Request
+    total = sum(len(a) for a in arrays)
+    out = (Request * total)()
+    size = ctypes.sizeof(Request)
+    at = 0
+    for a in arrays:
+        ctypes.memmove(ctypes.addressof(out) + at * size, ctypes.addressof(a), len(a) * size)
+        at += len(a)
+    return out
+
+
+def fresh_batches(native, batches, count, plain_best):
+    """The batched mix end to end on batches the library has not seen: each
+    of `batches` mixes of `count` instances (SplitMix64 seeds 101, 102, ...)
+    is validated, relabelled and ordered on the host, uploaded, searched and
+    decoded -- once as one td16_batch call per batch, once as one
+    td16_batch_stream call over all of them (chunk = one batch: the host
+    prepares batch b + 1 and decodes batch b - 1 while batch b is
+    searched).  Every witness is verified afterwards."""
+    from td16_amd._lib import Result
+    mixes = []
+    for b in range(batches):
+        ns, ks, perms = native.gen_products(101 + b, 12, 16, 4, 9, count)
+        mixes.append(build_requests(native, ns, ks, perms))
+    out = (Result * count)()
+    native.batch(mixes[0], out=out)  # warm the host pool and buffers
+    single = []
+    verified = 0
+    for m in mixes:
+        run = native.batch(m, out=out)
+        single.append(run.stats)
+        verified += verify(native, m, out)[1]
+    totals = sorted(st.total_ms for st in single)
+    med = totals[len(totals) // 2]
+    pick = min(single, key=lambda st: abs(st.total_ms - med))
+    allreqs = concat_requests(mixes)
+    outall = (Result * len(allreqs))()
+    native.batch_stream(allreqs, count, out=outall)  # warm
+    runs = [native.batch_stream(allreqs, count, out=outall) for _ in range(3)]
+    sv = verify(native, allreqs, outall)[1]
+    best = min(runs, key=lambda r: r.stats.total_ms).stats
+    stream_rate = sv / (best.total_ms * 1e-3)
+    single_rate = verified / (sum(st.total_ms for st in single) * 1e-3)
+    res = {
+        "batches": batches,
+        "instances_per_batch": count,
+        "host_threads": host_threads(),
+        "single_call": {
+            "ms_per_batch_median": round(med, 4),
+            "prepare_ms": round(pick.prepare_ms, 4),
+            "upload_ms": round(pick.upload_ms, 4),
+            "kernel_ms": round(pick.kernel_ms, 4),
+            "decode_ms": round(pick.decode_ms, 4),
+            "verified": verified,
+            "witnesses_per_s": round(single_rate, 1),
+        },
+        "stream": {
+            "ms_per_batch": round(best.total_ms / batches, 4),
+            "prepare_ms_per_batch": round(best.prepare_ms / batches, 4),
+            "upload_ms_per_batch": round(best.upload_ms / batches, 4),
+            "kernel_ms_per_batch": round(best.kernel_ms / batches, 4),
+            "decode_ms_per_batch": round(best.decode_ms / batches, 4),
+            "verified": sv,
+            "witnesses_per_s": round(stream_rate, 1),
+        },
+    }
+    if plain_best:
+        res["speedup_vs_plain_single_call"] = round(single_rate / plain_best, 2)
+        res["speedup_vs_plain_stream"] = round(stream_rate / plain_best, 2)
+    return res
+
+
+def config1(native, plain=True):
+    """Configuration 1: the exact transposition distance of 10,000 uniform
+    permutations of n = 10 (SplitMix64 seed 0) by iterative deepening over all
+    C(11,3) = 165 triples -- one batched launch, and the plain method (one
+    launch per instance) at 1 and 7 slots; the reference filter too.  Every
+    distance is compared with the oracle's, committed in tests/golden."""
+    exp = golden()["config1_distances"]
+    perms = native.gen_uniform(0, 10, 10000)
+    out = {"instances": len(perms)}
+    for ms, key in ((native.MOVES_ALL, "all"), (native.MOVES_REF, "ref")):
+        reqs = (native.Request * len(perms))(
+            *[native.linear_request(int(p), 10, 9, ms, native.FIRST, True) for p in perms])
+        native.batch(reqs)
+        run = native.batch(reqs, repeats=5)
+        st = run.stats
+        f = native.result_fields(run.results)
+        want = np.frombuffer(exp[key].encode(), dtype=np.uint8) - ord("0")
+        leg = {
+            "batched_kernel_ms": round(st.kernel_ms, 4),
+            "batched_instances_per_s": round(len(perms) / (st.wall_ms * 1e-3), 1),
+            "nodes_per_s": round(st.expanded / (st.kernel_ms * 1e-3), 1),
+            "distances_equal_oracle": int(((f["found"] == 1) & (f["depth"] == want)).sum()),
+        }
+        if plain and key == "all":
+            for slots in (1, 7):
+                rate, dt, res = plain_run(native, reqs, len(reqs), slots)
+                depth = np.array([r.depth for r in res])
+                leg["plain_%dslot_instances_per_s" % slots] = round(rate, 1)
+                leg["plain_%dslot_distances_equal_oracle" % slots] = int((depth == want).sum())
+        out[key] = leg
+    return out
+
+
+def config3(native):
+    """Configuration 3: 2,000 n = 16 instances (1,999 products of 9 block
+    transpositions, SplitMix64 seed 2, and the reverse permutation) at move
+    bound 8 under the reference filter, in one batched launch; found /
+    not-found against the oracle's for all 2,000 and the complete trees of
+    the non-existence proofs node for node.  Then the exhaustive depth-8
+    searches (COUNT_ALL, all moves) of the zero-slack instances whose trees
+    the CPU finishes, node and solution counts against the oracle's."""
+    exp = golden()
+    _, _, p3 = native.gen_products(2, 16, 16, 9, 9, 1999)
+    rev = sum(x << (4 * (15 - x)) for x in range(16))
+    perms = list(p3) + [rev]
+    reqs = (native.Request * 2000)(*[native.linear_request(int(p), 16, 8) for p in perms])
+    native.batch(reqs, waves_per_cu=20)
+    run = native.batch(reqs, waves_per_cu=20, repeats=3)
+    st = run.stats
+    f = native.result_fields(run.results)
+    want = np.frombuffer(exp["config3_found"].encode(), dtype=np.uint8) - ord("0")
+    found, verified = verify(native, reqs, run.results)
+    proofs = exp["config3_nonexistence"]
+    proofs_exact = sum(1 for p in proofs
+                       if run.results[p["q"]].found == 0
+                       and run.results[p["q"]].expanded == p["expanded"])
+    rows = exp["config3_count_all"]["rows"]
+    creqs = (native.Request * len(rows))(
+        *[native.linear_request(r["perm"], 16, 8, native.MOVES_ALL, native.COUNT_ALL)
+          for r in rows])
+    native.batch(creqs, waves_per_cu=20)
+    crun = native.batch(creqs, waves_per_cu=20, repeats=3)
+    exact = sum(1 for q, r in enumerate(rows)
+                if (crun.results[q].expanded, crun.results[q].solutions)
+                == (r["expanded"], r["solutions"]))
+    return {
+        "instances": 2000,
+        "kernel_ms": round(st.kernel_ms, 4),
+        "instances_per_s": round(2000 / (st.wall_ms * 1e-3), 1),
+        "nodes_per_s": round(st.expanded / (st.kernel_ms * 1e-3), 1),
+        "found": found,
+        "verified": verified,
+        "found_equal_oracle": int((f["found"] == want).sum()),
+        "nonexistence_proofs": len(proofs),
+        "proofs_node_counts_exact": proofs_exact,
+        "exhaustive_zero_slack": {
+            "instances": len(rows),
+            "of_zero_slack": exp["config3_count_all"]["zero_slack"],
+            "cpu_cap_nodes": exp["config3_count_all"]["cap"],
+            "nodes": crun.stats.expanded,
+            "kernel_ms": round(crun.stats.kernel_ms, 4),
+            "nodes_per_s": round(crun.stats.expanded / (crun.stats.kernel_ms * 1e-3), 1),
+            "counts_equal_oracle": exact,
+        },
     }
 
 
@@ -295,7 +471,8 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--emitter-instances", type=int, default=1000000)
     ap.add_argument("--skip", default="",
-                    help="comma list of: tune,plain,served,cpu,probe,cert,emitter,stage")
+                    help="comma list of: tune,plain,fresh,config1,config3,served,cpu,probe,"
+                         "cert,emitter,stage")
     ap.add_argument("--knobs", default="", metavar="W,S,D,C",
                     help="waves per CU, split depth, donate max, check interval for the "
                          "batched launch instead of tuning them")
@@ -376,7 +553,18 @@ def main() -> int:
             extra["plain_%dslot_witnesses_per_s" % slots] = round(rate, 1)
             best = rate if best is None else max(best, rate)
         plain = best
-        extra["speedup_vs_plain"] = round(value / plain, 2) if plain else None
+        # the headline re-runs requests already prepared and resident in HBM;
+        # the fresh-batch leg below divides end-to-end rates by the same plain rate
+        extra["speedup_vs_plain_resident"] = round(value / plain, 2) if plain else None
+    if "fresh" not in skip:
+        extra["fresh_batches"] = fresh_batches(native, 8, args.instances, plain)
+        log("fresh batches", extra["fresh_batches"])
+    if "config1" not in skip:
+        extra["config1"] = config1(native, plain="plain" not in skip)
+        log("config1", extra["config1"])
+    if "config3" not in skip:
+        extra["config3"] = config3(native)
+        log("config3", extra["config3"])
     if "served" not in skip:
         extra["served"] = served_latency(native, args.served_rate, args.served_requests,
                                          args.served_slots, 512, 1 << 30)

@@ -177,7 +177,9 @@ typedef struct {
     int64_t queue_bytes;     /* HBM for the shared work queue; 0 = 256 MiB */
     int32_t check_every;     /* node expansions between an item's sharing
                                 checks (at most 1024); 0 = default (16) */
-    int32_t reserved;        /* 0 */
+    int32_t host_threads;    /* host threads that prepare the requests and decode
+                                the results; 0 = default (all hardware threads,
+                                at most 16) */
 } td16_batch_options;
 
 typedef struct {
@@ -190,6 +192,12 @@ typedef struct {
     uint64_t donations;      /* work items pushed to the shared queue */
     int32_t waves;           /* wavefronts in the persistent grid */
     int32_t blocks;
+    double prepare_ms;       /* host: validation, relabelling and feed order of
+                                the requests (td16_batch; 0 for re-runs of
+                                requests prepared once) */
+    double upload_ms;        /* device: the request table's copy to HBM */
+    double decode_ms;        /* host: results of the last repeat to td16_result */
+    double total_ms;         /* host wall time of the whole call */
 } td16_batch_stats;
 
 /* Runs `count` requests in one persistent, work-sharing launch per repeat.
@@ -197,6 +205,22 @@ typedef struct {
 int td16_batch(const td16_request *reqs, int count, td16_result *results,
                const td16_batch_options *opt, td16_batch_stats *stats,
                char *msg, int msg_cap);
+
+/* A stream of requests as consecutive persistent launches of at most
+ * `chunk` requests each, double-buffered: while chunk c is searched, the
+ * host prepares chunk c + 1 (validation, relabelling, feed order) and
+ * decodes chunk c - 1's results, and chunk c + 1's upload runs beside the
+ * search on a second stream.  This is how a producer that hands over fresh
+ * batches keeps the device busy; td16_batch pays the same host work before
+ * and after its one launch.  Results and options as td16_batch (use_graph
+ * and repeats do not apply); in stats the times are sums over the chunks
+ * (wall_ms: the host's waits for searches to end) and total_ms the wall
+ * time of the whole call.  The reference searches one instance per call
+ * (sorting_search.cu:811-972); the Java producers stay the source of the
+ * stream. */
+int td16_batch_stream(const td16_request *reqs, int count, int chunk, td16_result *results,
+                      const td16_batch_options *opt, td16_batch_stats *stats,
+                      char *msg, int msg_cap);
 
 /* Allocation-free variant of td16_batch.  It has no counterpart in the
  * reference: it is for callers that cannot allocate device memory -- code
@@ -264,16 +288,27 @@ int td16_autotune(const td16_request *reqs, int count, td16_result *results,
 
 typedef struct {
     int32_t device;
-    int32_t slots;           /* requests admitted on the device at once (7..4096) */
+    int32_t slots;           /* requests admitted on the device at once (1..65536) */
     int32_t queue_capacity;  /* requests waiting for a slot (the reference's
                                 tdp.gpuQueueCapacity, GPUSortingSearch.java:82-89) */
     int32_t waves_per_cu;    /* resident wavefronts per CU; 0 = 4 (tuned for latency) */
-    int64_t slot_bytes;      /* HBM budget per slot (tdp.gpuBudgetMB) */
+    int64_t slot_bytes;      /* HBM budget per slot (tdp.gpuBudgetMB): the loop
+                                allocates slots x slot_bytes for queued subtrees
+                                (TD16_OUT_OF_MEMORY beyond the device's free
+                                memory) and an admitted request keeps at most
+                                slot_bytes of them queued (48-byte subtree plus
+                                8-byte ring word each; a full budget only stops
+                                its donations, the work stays with the
+                                wavefronts that hold it) */
+    int32_t deadline_ms;     /* 0 = none; else td16_serve_wait and td16_serve_stop
+                                give up after this long with TD16_DEVICE_ERROR */
+    int32_t reserved;        /* 0 */
 } td16_serve_options;
 
 typedef struct {
     uint64_t submitted, completed, rejected;
-    double p50_us, p99_us, mean_us; /* submit -> result visible on the host */
+    double p50_us, p99_us, mean_us; /* submit -> result visible on the host; p50 and
+                                       p99 to 2 % (log-spaced buckets), mean exact */
     int32_t in_flight, waiting;
 } td16_serve_stats;
 
@@ -283,10 +318,14 @@ typedef struct {
 int td16_serve_start(const td16_serve_options *opt, char *msg, int msg_cap);
 /* Queues a request; blocks while `queue_capacity` requests are waiting
  * (GPUSortingSearch.java:216 blocks the same way) unless nonblocking, in
- * which case TD16_QUEUE_FULL is returned.  *ticket identifies the request. */
+ * which case TD16_QUEUE_FULL is returned.  *ticket identifies the request.
+ * Once td16_serve_stop has begun, new submits fail with
+ * TD16_NOT_INITIALIZED (a submit blocked on a full queue too). */
 int td16_serve_submit(const td16_request *req, int nonblocking, uint64_t *ticket,
                       char *msg, int msg_cap);
-/* Blocks until the ticket's result is ready and returns it. */
+/* Blocks until the ticket's result is ready and returns it.  Returns
+ * TD16_DEVICE_ERROR when the served kernel has failed or ended, or when the
+ * deadline passes (the ticket then stays valid for a later wait). */
 int td16_serve_wait(uint64_t ticket, td16_result *res, char *msg, int msg_cap);
 /* Submit + wait. */
 int td16_serve_search(const td16_request *req, td16_result *res, char *msg, int msg_cap);
@@ -298,7 +337,16 @@ int td16_serve_stats_get(td16_serve_stats *st, int reset_latency, char *msg, int
  * the host.  *elapsed_us receives the replay's wall time. */
 int td16_serve_replay(const td16_request *reqs, const double *arrival_us, int count,
                       td16_result *results, double *elapsed_us, char *msg, int msg_cap);
-/* Stops the kernel after the admitted requests finish. */
+/* Closed-loop load: `producers` host threads (1..1024) each submit a request
+ * and wait for its result, then take the next (producer p takes reqs[p],
+ * reqs[p + producers], ...), with no pacing -- the loop's capacity, as many
+ * producer threads calling GPUSortingSearch.search would see it
+ * (GPUSortingSearch.java:203-240).  Resets the latency window first. */
+int td16_serve_closed_loop(const td16_request *reqs, int count, int producers,
+                           td16_result *results, double *elapsed_us, char *msg, int msg_cap);
+/* Stops the kernel after the admitted requests finish.  With a deadline, a
+ * loop that does not drain and stop in time (or whose kernel failed) returns
+ * TD16_DEVICE_ERROR and leaves its device memory allocated. */
 int td16_serve_stop(char *msg, int msg_cap);
 
 /* ---------------------------------------------------------------------- */

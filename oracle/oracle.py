@@ -94,6 +94,10 @@ def lib():
         L.td16o_search_batch.argtypes = [ctypes.c_int, u8p, u64p, u8p, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(Result)]
+        L.td16o_search_batch_ex.argtypes = [ctypes.c_int, u8p, u64p, u8p, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_uint64, ctypes.c_int,
+                                            ctypes.POINTER(Result)]
         _lib = L
     return _lib
 
@@ -285,4 +289,20 @@ def search_batch(ns, perms, ks, moveset=MOVES_REF, mode=FIRST, threads=1, greedy
                              perms.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                              ks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                              moveset, mode, int(greedy), threads, out)
+    return out
+
+
+def search_batch_ex(ns, perms, ks, moveset=MOVES_REF, mode=FIRST, threads=1, greedy=False,
+                    iddfs=False, cap=0):
+    """search_batch with iterative deepening (exact distance within k) and a
+    node cap (found = -2 once a search passes `cap` expanded nodes)."""
+    count = len(ns)
+    out = (Result * max(1, count))()
+    ns = np.ascontiguousarray(ns, dtype=np.uint8)
+    perms = np.ascontiguousarray(perms, dtype=np.uint64)
+    ks = np.ascontiguousarray(ks, dtype=np.uint8)
+    lib().td16o_search_batch_ex(count, ns.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                perms.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                ks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                moveset, mode, int(greedy), int(iddfs), int(cap), threads, out)
     return out

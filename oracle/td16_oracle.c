@@ -749,7 +749,8 @@ typedef struct {
     const uint8_t *ns;
     const uint64_t *perms;
     const uint8_t *ks;
-    int moveset, mode, greedy;
+    int moveset, mode, greedy, iddfs;
+    uint64_t cap;
     td16o_result *out;
     int next;
     pthread_mutex_t mu;
@@ -771,8 +772,16 @@ static void *batch_worker(void *arg) {
         /* caller convention E/UnorientedCycles.java:34 */
         const float minRate = 2.0f / ((float)k / (float)target);
         td16o_result *r = &b->out[idx];
-        td16o_search_order(beta, omega, N, odd, minRate, k, 1, b->moveset, b->mode, b->greedy, r);
-        if (r->found) {
+        if (b->iddfs) {
+            /* exact distance within k (depth 0 for a sorted input) */
+            td16o_iddfs_order(beta, omega, N, b->moveset, k, b->greedy, r);
+            if (odd == N) r->found = 1;
+        } else {
+            uint32_t sat[10], reach[10];
+            td16o_rate_tables(N, odd, minRate, k, 1, sat, reach);
+            search_impl(beta, omega, N, k, sat, reach, b->moveset, b->mode, b->greedy, b->cap, r);
+        }
+        if (r->found == 1 && r->depth > 0) {
             int32_t sym[27];
             td16o_positions_to_symbols(beta, N, &r->moves[0][0], r->depth, sym);
             if (td16o_validate(beta, omega, N, sym, r->depth, k) != 0) r->found = -1;
@@ -783,6 +792,13 @@ static void *batch_worker(void *arg) {
 
 int td16o_search_batch(int count, const uint8_t *ns, const uint64_t *perms, const uint8_t *ks,
                        int moveset, int mode, int greedy, int threads, td16o_result *out) {
+    return td16o_search_batch_ex(count, ns, perms, ks, moveset, mode, greedy, 0, 0, threads,
+                                 out);
+}
+
+int td16o_search_batch_ex(int count, const uint8_t *ns, const uint64_t *perms,
+                          const uint8_t *ks, int moveset, int mode, int greedy, int iddfs,
+                          uint64_t cap, int threads, td16o_result *out) {
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;
     batch_ctx b;
@@ -793,6 +809,8 @@ int td16o_search_batch(int count, const uint8_t *ns, const uint64_t *perms, cons
     b.moveset = moveset;
     b.mode = mode;
     b.greedy = greedy;
+    b.iddfs = iddfs;
+    b.cap = cap;
     b.out = out;
     b.next = 0;
     pthread_mutex_init(&b.mu, NULL);

@@ -180,6 +180,15 @@ int td16o_search_batch(int count, const uint8_t *ns, const uint64_t *perms,
                        const uint8_t *ks, int moveset, int mode, int greedy,
                        int threads, td16o_result *out);
 
+/* The same with two more choices: iddfs = 1 computes the exact distance
+ * within k by iterative deepening (found = 1, depth = distance; depth 0 for
+ * a sorted input), and cap > 0 stops a search after `cap` expanded nodes
+ * (found = -2: the tree is larger than the cap).  A witness that fails the
+ * replay turns found into -1. */
+int td16o_search_batch_ex(int count, const uint8_t *ns, const uint64_t *perms,
+                          const uint8_t *ks, int moveset, int mode, int greedy, int iddfs,
+                          uint64_t cap, int threads, td16o_result *out);
+
 #ifdef __cplusplus
 }
 #endif

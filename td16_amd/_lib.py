@@ -98,7 +98,7 @@ class BatchOptions(ctypes.Structure):
         ("repeats", ctypes.c_int32),
         ("queue_bytes", ctypes.c_int64),
         ("check_every", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("host_threads", ctypes.c_int32),
     ]
 
 
@@ -111,6 +111,10 @@ class BatchStats(ctypes.Structure):
         ("donations", ctypes.c_uint64),
         ("waves", ctypes.c_int32),
         ("blocks", ctypes.c_int32),
+        ("prepare_ms", ctypes.c_double),
+        ("upload_ms", ctypes.c_double),
+        ("decode_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
     ]
 
 
@@ -121,6 +125,8 @@ class ServeOptions(ctypes.Structure):
         ("queue_capacity", ctypes.c_int32),
         ("waves_per_cu", ctypes.c_int32),
         ("slot_bytes", ctypes.c_int64),
+        ("deadline_ms", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -164,6 +170,8 @@ SIGNATURES = {
     "td16_search_ex": (_i, [_i, ctypes.POINTER(Request), ctypes.POINTER(Result), _c, _i]),
     "td16_batch": (_i, [ctypes.POINTER(Request), _i, ctypes.POINTER(Result),
                         ctypes.POINTER(BatchOptions), ctypes.POINTER(BatchStats), _c, _i]),
+    "td16_batch_stream": (_i, [ctypes.POINTER(Request), _i, _i, ctypes.POINTER(Result),
+                               ctypes.POINTER(BatchOptions), ctypes.POINTER(BatchStats), _c, _i]),
     "td16_batch_noalloc": (_i, [ctypes.POINTER(Request), _i, ctypes.POINTER(Result),
                                 ctypes.POINTER(BatchOptions), ctypes.POINTER(BatchStats), _c, _i]),
     "td16_autotune": (_i, [ctypes.POINTER(Request), _i, ctypes.POINTER(Result),
@@ -175,6 +183,8 @@ SIGNATURES = {
     "td16_serve_search": (_i, [ctypes.POINTER(Request), ctypes.POINTER(Result), _c, _i]),
     "td16_serve_stats_get": (_i, [ctypes.POINTER(ServeStats), _i, _c, _i]),
     "td16_serve_stop": (_i, [_c, _i]),
+    "td16_serve_closed_loop": (_i, [ctypes.POINTER(Request), _i, _i, ctypes.POINTER(Result),
+                                    ctypes.POINTER(ctypes.c_double), _c, _i]),
     "td16_serve_replay": (_i, [ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double), _i,
                                ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_double), _c, _i]),
     "td16_probe_valu_peak": (_i, [_i, _i, ctypes.POINTER(ctypes.c_double), _c, _i]),

@@ -787,6 +787,12 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
         // heavy requests spread early instead of crawling on one wavefront
         // that shares its SIMD with light ones
         int64_t room = drained ? -bal : (c.expanded >= heavyAt ? kEarlyStock - bal : 0);
+        if (P.itemCap > 0) {
+          // a served request keeps at most its slot's budget of items pending
+          const int64_t pend = c.everShared ? ld_rlx(&P.states[c.job].pending) : 1;
+          const int64_t left = (int64_t)P.itemCap - pend;
+          if (left < room) room = left;
+        }
         if (room <= 0) return 0;
         return room < (int64_t)P.donateMax ? (int)room : P.donateMax;
       });

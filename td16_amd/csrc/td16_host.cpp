@@ -120,13 +120,55 @@ int rate_tables(int N, int initialEven, float minRate, int maxDepth, int fullSor
   return 0;
 }
 
+namespace {
+
+// Callers without a memo of their own share one per thread.
+thread_local RateMemo tMemo;
+
+const RateMemo::Entry &rate_entry(RateMemo &memo, int N, int initialEven, float minRate,
+                                  int maxMoves, int full, bool iddfs) {
+  uint32_t rb;
+  std::memcpy(&rb, &minRate, sizeof(rb));
+  const uint64_t k1 = (uint64_t)rb | ((uint64_t)(uint8_t)N << 32) |
+                      ((uint64_t)(uint8_t)maxMoves << 40) | ((uint64_t)(full != 0) << 48) |
+                      ((uint64_t)iddfs << 49);
+  const uint64_t k2 = (uint64_t)(uint32_t)initialEven;
+  uint64_t h = (k1 ^ (k2 * 0x9E3779B97F4A7C15ULL)) * 0xBF58476D1CE4E5B9ULL;
+  RateMemo::Entry &e = memo.e[(h >> 40) % RateMemo::kEntries];
+  if (e.k1 == k1 && e.k2 == k2) return e;
+  rate_tables(N, initialEven, minRate, maxMoves, full, e.sat, e.reach);
+  // No slack: the odd-cycle count rises by at most two per move, so depth
+  // d reaches initial_even .. initial_even + 2d; when the first depth with a
+  // witness among those is the move bound itself, every move must be
+  // perfect.  Long FIRST searches come from such requests.
+  e.tight = false;
+  if (!iddfs) {
+    for (int d = 0; d <= maxMoves && d <= kMaxDepth; ++d) {
+      uint32_t odds = 0;  // odd-cycle counts depth d can reach
+      for (int i = 0; i <= d; ++i) {
+        const int ev = initialEven + 2 * i;
+        if (ev >= 0 && ev < 32) odds |= 1u << ev;
+      }
+      if (e.sat[d] & odds) {
+        e.tight = d == maxMoves;
+        break;
+      }
+    }
+  }
+  e.k1 = k1;
+  e.k2 = k2;
+  return e;
+}
+
+}  // namespace
+
 // Validates a request (emitter/src/native/backtracking/sorting_search.cu:
 // 820-825 and jni_sorter.cu:61-67) and turns it into the kernel's label form.
 // Returns TD16_OK, or TD16_INVALID_ARGUMENT with a message.  *trivial is set
 // when no launch is needed: n < 3 (no triple: sorting_search.cu:826-827) or
 // an IDDFS request whose first limit exceeds max_moves or that is sorted.
-int make_job_checked(const td16_request *r, Job *job, int *trivial, char *msg, int cap) {
-  std::memset(job, 0, sizeof(*job));
+int make_job(const td16_request *r, Job *job, int *trivial, RateMemo &memo, char *msg,
+             int cap) {
   *trivial = 0;
   const int N = r->n;
   if (N < 0 || N > kMaxN)
@@ -138,26 +180,34 @@ int make_job_checked(const td16_request *r, Job *job, int *trivial, char *msg, i
     return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "unknown move set %d", r->moveset);
   if (r->mode != TD16_FIRST && r->mode != TD16_COUNT_ALL)
     return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "unknown search mode %d", r->mode);
-  if (!is_perm(r->beta, N) || !is_perm(r->omega, N))
+  // both are permutations of 0..N-1 iff every value is below N and the N
+  // values cover all N bits
+  uint32_t sb = 0, so = 0, over = 0;
+  for (int p = 0; p < N; ++p) {
+    const uint32_t b = r->beta[p], o = r->omega[p];
+    over |= (uint32_t)(b >= (uint32_t)N) | (uint32_t)(o >= (uint32_t)N);
+    sb |= 1u << (b & 31);
+    so |= 1u << (o & 31);
+  }
+  const uint32_t full = (1u << N) - 1u;
+  if (over || sb != full || so != full)
     return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
                    "beta and omega must be permutations of 0..n-1");
+  std::memset(job, 0, sizeof(*job));
   if (N < 3) {
     *trivial = 1;
     return TD16_OK;
   }
-  uint8_t lab[kMaxN], next[kMaxN];
-  for (int p = 0; p < N; ++p) {
-    lab[r->beta[p]] = (uint8_t)p;
-    next[r->beta[p]] = r->beta[(p + 1) % N];
-  }
-  // rho = omega o beta (sorting_search.cu:899-913); R[q] = label(rho(beta[q]))
-  for (int q = 0; q < N; ++q) job->R[q] = lab[r->omega[next[r->beta[q]]]];
-  uint64_t pi = ~0ULL;
-  for (int p = 0; p < N - 1; ++p) {
-    pi &= ~(0xFULL << (4 * p));
-    pi |= (uint64_t)p << (4 * p);
-  }
-  job->pi0 = pi;
+  uint8_t lab[32];
+  for (int p = 0; p < N; ++p) lab[r->beta[p]] = (uint8_t)p;
+  // rho = omega o beta (sorting_search.cu:899-913): R[q] = label(rho(beta[q]))
+  // and beta's function form maps beta[q] to beta[q + 1]
+  for (int q = 0; q + 1 < N; ++q) job->R[q] = lab[r->omega[r->beta[q + 1]]];
+  job->R[N - 1] = lab[r->omega[r->beta[0]]];
+  // labels 0..N-2 at positions 0..N-2, unused nibbles 0xF
+  const int used = 4 * (N - 1);  // 8..64 bits
+  job->pi0 = used >= 64 ? 0xFEDCBA9876543210ULL
+                        : (0xFEDCBA9876543210ULL & ((1ULL << used) - 1ULL)) | (~0ULL << used);
   job->N = (uint8_t)N;
   job->maxDepth = (uint8_t)r->max_moves;
   job->flags = (uint8_t)((r->moveset == TD16_MOVES_ALL ? kFlagAllMoves : 0) |
@@ -179,29 +229,21 @@ int make_job_checked(const td16_request *r, Job *job, int *trivial, char *msg, i
       return TD16_OK;
     }
     job->firstLimit = (uint8_t)first;
-    rate_tables(N, odd, 0.0f, r->max_moves, 1, job->sat, job->reach);
+    const RateMemo::Entry &e = rate_entry(memo, N, odd, 0.0f, r->max_moves, 1, true);
+    std::memcpy(job->sat, e.sat, sizeof(job->sat));
+    std::memcpy(job->reach, e.reach, sizeof(job->reach));
   } else {
-    rate_tables(N, r->initial_even, r->min_rate, r->max_moves, r->full_sorting, job->sat,
-                job->reach);
-    // No slack: the odd-cycle count rises by at most two per move, so depth
-    // d reaches initial_even .. initial_even + 2d; when the first depth with a
-    // witness among those is the move bound itself, every move must be
-    // perfect.  Long FIRST searches come from such requests.
-    if (r->mode == TD16_FIRST) {
-      for (int d = 0; d <= r->max_moves && d <= kMaxDepth; ++d) {
-        uint32_t odds = 0;  // odd-cycle counts depth d can reach
-        for (int i = 0; i <= d; ++i) {
-          const int e = r->initial_even + 2 * i;
-          if (e >= 0 && e < 32) odds |= 1u << e;
-        }
-        if (job->sat[d] & odds) {
-          if (d == r->max_moves) job->flags |= kFlagTight;
-          break;
-        }
-      }
-    }
+    const RateMemo::Entry &e = rate_entry(memo, N, r->initial_even, r->min_rate, r->max_moves,
+                                          r->full_sorting, false);
+    std::memcpy(job->sat, e.sat, sizeof(job->sat));
+    std::memcpy(job->reach, e.reach, sizeof(job->reach));
+    if (r->mode == TD16_FIRST && e.tight) job->flags |= kFlagTight;
   }
   return TD16_OK;
+}
+
+int make_job_checked(const td16_request *r, Job *job, int *trivial, char *msg, int cap) {
+  return make_job(r, job, trivial, tMemo, msg, cap);
 }
 
 void unpack_moves(const uint16_t *packed, int depth, int32_t (*moves)[3]) {

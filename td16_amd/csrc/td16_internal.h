@@ -195,6 +195,7 @@ struct Params {
   int32_t served;        // 0 batched, 1 served loop
   int32_t idleBackoff;   // longest idle sleep, in units of s_sleep(16)
   int32_t checkEvery;    // expansions between an item's sharing checks
+  int32_t itemCap;       // served: most work items one request may have pending (0 = no cap)
   uint32_t ringCap;      // served: power of two
   ServeHost *host;       // served: device view of the host block
   const Job *hostJobs;   // served: device view of the host submit ring
@@ -225,10 +226,59 @@ void batch_grid(int wavesPerCU, int cus, int blocksPerCU, int blocksPerCUWide, i
                 bool *wide);
 void decode_result(const DevResult &r, td16_result *out);
 void trivial_result(int trivial, td16_result *out);
+
+// The witness / prune tables and the no-slack flag depend only on (N,
+// initial_even, min_rate, max_moves, full_sorting) -- and on whether the
+// request is an iterative-deepening one -- and a batch has few distinct such
+// keys (the batched mix: about 300 among 100,000 requests), so the tables
+// last built are kept in a small direct-mapped memo, one per preparing
+// thread.
+struct RateMemo {
+  static constexpr int kEntries = 512;
+  struct Entry {
+    uint64_t k1 = ~0ULL, k2 = ~0ULL;
+    uint32_t sat[10], reach[10];
+    bool tight;  // FIRST search without slack (kFlagTight)
+  } e[kEntries];
+};
+// Validates one request and writes its label form (td16_host.cpp); *trivial
+// is set when no launch is needed.  make_job_checked uses a per-thread memo.
+int make_job(const td16_request *r, Job *job, int *trivial, RateMemo &memo, char *msg, int cap);
+int make_job_checked(const td16_request *r, Job *job, int *trivial, char *msg, int cap);
+
+// Host preparation and decoding of a batch (td16_prepare.cpp), on a pool of
+// host threads.  host_threads(0) is the default thread count.
+int host_threads(int want);
+// Position of the r-th request of the feed order once the sorted list is
+// dealt round-robin to the feed shards of `per` requests (init_feed).
+uint32_t deal_position(uint32_t r, uint32_t nj, uint32_t per);
+struct PrepareScratch {
+  std::vector<Job> jobs;
+  std::vector<uint16_t> keys;    // per request: its feed bucket
+  std::vector<uint32_t> pos;     // per request: its feed position, or kNotLaunched
+  std::vector<uint32_t> counts;
+  std::vector<RateMemo> memos;  // one per part
+};
 // Validates and relabels every request, answers the trivial ones in
-// `results` and returns the rest in feed order (`where[q]` is the caller's
-// index of jobs[q]).
+// `results` and writes the rest in feed order to jobsOut[0 .. *njOut)
+// (`whereOut[q]` is the caller's index of jobsOut[q]); both arrays hold
+// `count` entries.  An invalid request stops the batch: its result gets the
+// status and `msg` the reason.
+int prepare_batch_into(const td16_request *reqs, int count, td16_result *results, Job *jobsOut,
+                       int *whereOut, uint32_t *njOut, PrepareScratch &ws, int threads,
+                       char *msg, int cap);
+// The same into vectors sized to the requests that need a launch.
 int prepare_batch(const td16_request *reqs, int count, td16_result *results,
                   std::vector<Job> &jobs, std::vector<int> &where, char *msg, int cap);
+// Batched results sit at their requests' feed positions; pos[q] is request
+// q's position (kNotLaunched for a request answered on the host).
+// decode_batch fills results[q] from dev[pos[q]] for every launched q <
+// count, walking the caller's order, and sums the counters.
+constexpr uint32_t kNotLaunched = 0xFFFFFFFFu;
+void decode_batch(const DevResult *dev, const uint32_t *pos, uint32_t count,
+                  td16_result *results, int threads, uint64_t *expanded, uint64_t *solutions,
+                  uint64_t *donations);
+// pos for decode_batch from the caller indices of the launched requests.
+void positions_of(const int *where, uint32_t nj, uint32_t count, std::vector<uint32_t> &pos);
 
 }  // namespace td16

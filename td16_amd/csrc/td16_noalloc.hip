@@ -182,12 +182,10 @@ extern "C" int td16_batch_noalloc(const td16_request *reqs, int count, td16_resu
   NA_HIP(hipMemcpy(out.data(), P.results, sizeof(DevResult) * nj, hipMemcpyDeviceToHost));
   local.kernel_ms = wallMs / o.repeats;  // host-timed: no events are created
   local.wall_ms = wallMs / o.repeats;
-  for (uint32_t q = 0; q < nj; ++q) {
-    decode_result(out[q], &results[where[q]]);
-    local.expanded += out[q].expanded;
-    local.solutions += out[q].solutions;
-    local.donations += (uint64_t)out[q].split;
-  }
+  std::vector<uint32_t> pos;
+  positions_of(where.data(), nj, (uint32_t)count, pos);
+  decode_batch(out.data(), pos.data(), (uint32_t)count, results, o.host_threads, &local.expanded,
+               &local.solutions, &local.donations);
   if (stats) *stats = local;
   return TD16_OK;
 }

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -16,6 +17,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -200,24 +202,6 @@ void batch_grid(int wavesPerCU, int cus, int blocksPerCU, int blocksPerCUWide, i
   *blocks = cus * std::max(1, perCU);
 }
 
-void decode_result(const DevResult &r, td16_result *out) {
-  std::memset(out, 0, sizeof(*out));
-  out->status = TD16_OK;
-  out->found = r.found;
-  out->depth = r.found ? r.depth : 0;
-  out->split = r.split != 0 ? 1 : 0;
-  out->expanded = r.expanded;
-  out->solutions = r.solutions;
-  out->device_us = (r.t1 >= r.t0) ? (double)(r.t1 - r.t0) * 0.01 : 0.0;  // 100 MHz ticks
-  unpack_moves(r.moves, out->depth, out->moves);
-}
-
-void trivial_result(int trivial, td16_result *out) {
-  std::memset(out, 0, sizeof(*out));
-  out->status = TD16_OK;
-  if (trivial == 2) out->found = 1;  // IDDFS on a sorted input: distance 0
-}
-
 namespace {
 
 Params make_params(const Buffers &b, const DeviceInfo &dev) {
@@ -316,7 +300,9 @@ int run_plain(Slot &s, const td16_request *req, td16_result *res, char *msg, int
 struct BatchCtx {
   int device = -1;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evUp = nullptr, ev0 = nullptr, ev1 = nullptr;
+  PrepareScratch ws;
+  std::vector<int> where;   // caller index of each prepared request
   Buffers buf;
   int64_t queueBytes = 0;
   hipGraphExec_t graph = nullptr;
@@ -327,6 +313,36 @@ struct BatchCtx {
 std::mutex gBatchMu;
 BatchCtx gBatch;
 
+// Streamed batches: two launch contexts used in turn, so that the host
+// prepares chunk c + 1 into one while chunk c runs from the other; uploads
+// run on their own stream, beside the search.
+struct StreamCtx {
+  int device = -1;
+  hipStream_t run = nullptr, copy = nullptr;
+  hipEvent_t up0[2] = {}, up1[2] = {}, k0[2] = {}, k1[2] = {};
+  Buffers buf[2];
+  PrepareScratch ws[2];
+  std::vector<int> where[2];
+  int64_t queueBytes = 0;
+
+  void release() {
+    if (device >= 0) hipSetDevice(device);
+    if (run) hipStreamSynchronize(run);
+    if (copy) hipStreamSynchronize(copy);
+    for (int s = 0; s < 2; ++s) {
+      buf[s].release();
+      for (hipEvent_t e : {up0[s], up1[s], k0[s], k1[s]})
+        if (e) hipEventDestroy(e);
+    }
+    if (run) hipStreamDestroy(run);
+    if (copy) hipStreamDestroy(copy);
+    *this = StreamCtx();
+  }
+};
+
+std::mutex gStreamMu;
+StreamCtx gStream;
+
 void reset_batch_graph() {
   if (gBatch.graph) hipGraphExecDestroy(gBatch.graph);
   gBatch.graph = nullptr;
@@ -336,20 +352,73 @@ void reset_batch_graph() {
 // Served loop
 // ---------------------------------------------------------------------------
 
+// One submitted request.  Its result is handed over through `state` (0
+// pending, 1 ready, 2 failed with the device) and the ticket's own
+// condition variable, so a completion wakes exactly its waiter.
 struct Ticket {
-  bool ready = false;
+  std::atomic<int> state{0};
   td16_result res;
   std::chrono::steady_clock::time_point t0;      // arrival
   std::chrono::steady_clock::time_point tAdmit;  // published to the device
+  std::mutex m;
+  std::condition_variable cv;
+};
+using TicketPtr = std::shared_ptr<Ticket>;
+
+// Submit-to-result latency, in log-spaced buckets 2 % wide from 0.05 us to
+// about 30 s (the last bucket takes anything longer): constant memory and an
+// O(1) update however long the loop serves.
+struct LatencyHist {
+  static constexpr int kBuckets = 1024;
+  static constexpr double kLo = 0.05, kRatio = 1.02;
+  std::atomic<uint64_t> n[kBuckets];
+  std::atomic<uint64_t> count{0}, sumNs{0};
+
+  LatencyHist() { reset(); }
+  void reset() {
+    for (auto &b : n) b.store(0, std::memory_order_relaxed);
+    count.store(0, std::memory_order_relaxed);
+    sumNs.store(0, std::memory_order_relaxed);
+  }
+  void add(double us) {
+    int b = 0;
+    if (us > kLo) b = (int)(std::log(us / kLo) / std::log(kRatio));
+    b = std::min(std::max(b, 0), kBuckets - 1);
+    n[b].fetch_add(1, std::memory_order_relaxed);
+    count.fetch_add(1, std::memory_order_relaxed);
+    sumNs.fetch_add((uint64_t)(us * 1000.0), std::memory_order_relaxed);
+  }
+  // geometric middle of the bucket that holds the p-quantile
+  double quantile(double p) const {
+    const uint64_t total = count.load(std::memory_order_relaxed);
+    if (total == 0) return 0.0;
+    const uint64_t rank = (uint64_t)(p * (double)(total - 1)) + 1;
+    uint64_t seen = 0;
+    for (int b = 0; b < kBuckets; ++b) {
+      seen += n[b].load(std::memory_order_relaxed);
+      if (seen >= rank) return b == 0 ? kLo : kLo * std::pow(kRatio, b + 0.5);
+    }
+    return kLo * std::pow(kRatio, kBuckets);
+  }
+  double mean() const {
+    const uint64_t total = count.load(std::memory_order_relaxed);
+    return total ? (double)sumNs.load(std::memory_order_relaxed) * 1e-3 / (double)total : 0.0;
+  }
+};
+
+constexpr int kTicketShards = 64;
+struct TicketShard {
+  std::mutex mu;
+  std::unordered_map<uint64_t, TicketPtr> map;
 };
 
 struct Server {
-  bool running = false;
   int device = -1;
   DeviceInfo dev;
   hipStream_t stream = nullptr;
   Buffers buf;
   uint32_t slots = 0, ringCap = 0, queueCap = 0;
+  int deadlineMs = 0;
   ServeHost *hHost = nullptr;
   Job *hRing = nullptr;
   DevResult *hRes = nullptr;
@@ -359,40 +428,53 @@ struct Server {
   DevResult *dRes = nullptr;
   uint32_t *dComp = nullptr;
 
-  std::mutex mu;
-  std::condition_variable cvDone, cvSpace, cvWork;
-  std::deque<std::pair<uint64_t, Job>> waiting;
+  // admission: requests waiting for a slot and the slots' tickets
+  std::mutex admitMu;
+  std::condition_variable cvSpace, cvIdle, cvWork;
+  std::deque<std::pair<TicketPtr, Job>> waiting;
   std::vector<uint32_t> freeSlots;
-  std::vector<uint64_t> slotTicket;
-  std::unordered_map<uint64_t, Ticket> tickets;
-  uint64_t nextTicket = 1;
+  std::vector<TicketPtr> slotTicket;
   uint64_t submittedSeq = 0;
-  uint32_t inFlight = 0;
-  bool stopping = false;
+  bool stopping = false;  // written and read under admitMu
+  std::atomic<uint32_t> inFlight{0};
+
+  std::atomic<uint64_t> nextTicket{1};
+  TicketShard shards[kTicketShards];
   std::thread poller;
   std::atomic<bool> pollerExit{false};
+  // set once the served kernel is seen to have failed or ended unasked
+  std::atomic<bool> failed{false};
+  char failure[160] = {0};
 
-  uint64_t nSubmitted = 0, nCompleted = 0, nRejected = 0;
-  std::vector<double> latUs;
+  std::atomic<uint64_t> nSubmitted{0}, nCompleted{0}, nRejected{0};
+  LatencyHist hist;
+
+  TicketShard &shard(uint64_t t) { return shards[t % kTicketShards]; }
 };
 
-std::mutex gServeMu;  // start/stop
-Server *gServer = nullptr;
+// gServeMu orders start and stop; gServerPtrMu guards only the pointer, so a
+// stop that drains the loop never holds up the waits it is draining.
+std::mutex gServeMu, gServerPtrMu;
+std::shared_ptr<Server> gServer;
 
-// Publishes waiting requests while slots are free (caller holds S->mu).
+std::shared_ptr<Server> current_server() {
+  std::lock_guard<std::mutex> lk(gServerPtrMu);
+  return gServer;
+}
+
+// Publishes waiting requests while slots are free (caller holds admitMu).
 // Called by submit and by the completion poller, so no thread hop sits
 // between a free slot and the next request.
 void admit_locked(Server *S) {
   bool any = false;
-  while (!S->waiting.empty() && S->inFlight < S->slots) {
-    auto item = S->waiting.front();
+  while (!S->waiting.empty() && !S->freeSlots.empty()) {
+    auto item = std::move(S->waiting.front());
     S->waiting.pop_front();
     const uint32_t slot = S->freeSlots.back();
     S->freeSlots.pop_back();
-    S->slotTicket[slot] = item.first;
-    auto tk = S->tickets.find(item.first);
-    if (tk != S->tickets.end()) tk->second.tAdmit = std::chrono::steady_clock::now();
-    __atomic_fetch_add(&S->inFlight, 1u, __ATOMIC_RELAXED);
+    item.first->tAdmit = std::chrono::steady_clock::now();
+    S->slotTicket[slot] = std::move(item.first);
+    S->inFlight.fetch_add(1, std::memory_order_relaxed);
     const uint64_t r = S->submittedSeq++;
     Job job = item.second;
     job.tag = slot;
@@ -407,19 +489,77 @@ void admit_locked(Server *S) {
   }
 }
 
-// Completion poller: spins on the next completion record while requests are
-// in flight (a dedicated core buys microseconds of latency); with nothing in
-// flight it parks on cvWork, which admission signals.
+// Fails every request still in the loop once the device is lost: their
+// waiters return TD16_DEVICE_ERROR instead of waiting forever.
+void fail_all(Server *S, const char *why) {
+  bool expected = false;
+  if (!S->failed.compare_exchange_strong(expected, true)) return;
+  std::snprintf(S->failure, sizeof(S->failure), "%s", why);
+  std::vector<TicketPtr> lost;
+  {
+    std::lock_guard<std::mutex> lk(S->admitMu);
+    for (auto &t : S->slotTicket)
+      if (t) lost.push_back(std::move(t));
+    for (auto &w : S->waiting) lost.push_back(std::move(w.first));
+    S->waiting.clear();
+    S->inFlight.store(0);
+    S->cvSpace.notify_all();
+    S->cvIdle.notify_all();
+  }
+  for (auto &t : lost) {
+    std::lock_guard<std::mutex> l(t->m);
+    t->res.status = TD16_DEVICE_ERROR;
+    t->state.store(2, std::memory_order_release);
+    t->cv.notify_all();
+  }
+}
+
+// Completion poller: takes every completion record the device has written,
+// frees their slots and admits waiting requests under one short lock, then
+// hands each result to its own waiter.  It spins while requests are in
+// flight (a dedicated core buys microseconds of latency) and parks on cvWork
+// otherwise; about every 50 ms it checks that the served kernel still runs.
 void poll_loop(Server *S) {
   uint64_t c = 0;
+  struct Done {
+    uint32_t slot;
+    DevResult r;
+  };
+  std::vector<Done> done;
+  done.reserve(256);
+  std::vector<TicketPtr> tickets;
+  tickets.reserve(256);
+  auto lastCheck = std::chrono::steady_clock::now();
   while (!S->pollerExit.load(std::memory_order_acquire)) {
-    const uint32_t e = (uint32_t)(c & (S->ringCap - 1));
-    const uint32_t seq = __atomic_load_n(&S->hComp[2 * e + 1], __ATOMIC_ACQUIRE);
-    if (seq != (uint32_t)(c + 1)) {
-      if (__atomic_load_n(&S->inFlight, __ATOMIC_RELAXED) == 0) {
-        std::unique_lock<std::mutex> lk(S->mu);
+    done.clear();
+    while (done.size() < 256) {
+      const uint32_t e = (uint32_t)(c & (S->ringCap - 1));
+      const uint32_t seq = __atomic_load_n(&S->hComp[2 * e + 1], __ATOMIC_ACQUIRE);
+      if (seq != (uint32_t)(c + 1)) break;
+      Done d;
+      d.slot = __atomic_load_n(&S->hComp[2 * e], __ATOMIC_ACQUIRE);
+      // copied before the slot is freed: its next request reuses the record
+      std::memcpy(&d.r, &S->hRes[d.slot], sizeof(DevResult));
+      done.push_back(d);
+      ++c;
+    }
+    const auto now = std::chrono::steady_clock::now();
+    if (done.empty()) {
+      if (now - lastCheck > std::chrono::milliseconds(50)) {
+        lastCheck = now;
+        const hipError_t q = hipStreamQuery(S->stream);
+        if (q != hipErrorNotReady) {
+          char why[128];
+          std::snprintf(why, sizeof why, "the served kernel ended unexpectedly (%s)",
+                        hipGetErrorName(q));
+          fail_all(S, why);
+          return;
+        }
+      }
+      if (S->inFlight.load(std::memory_order_relaxed) == 0) {
+        std::unique_lock<std::mutex> lk(S->admitMu);
         S->cvWork.wait_for(lk, std::chrono::milliseconds(1), [&] {
-          return S->inFlight > 0 || S->pollerExit.load(std::memory_order_acquire);
+          return S->inFlight.load() > 0 || S->pollerExit.load(std::memory_order_acquire);
         });
       } else {
 #if defined(__x86_64__)
@@ -428,30 +568,34 @@ void poll_loop(Server *S) {
       }
       continue;
     }
-    const uint32_t slot = __atomic_load_n(&S->hComp[2 * e], __ATOMIC_ACQUIRE);
-    DevResult r;
-    std::memcpy(&r, &S->hRes[slot], sizeof(DevResult));
-    const auto now = std::chrono::steady_clock::now();
+    tickets.clear();
     {
-      std::lock_guard<std::mutex> lk(S->mu);
-      const uint64_t t = S->slotTicket[slot];
-      auto it = S->tickets.find(t);
-      if (it != S->tickets.end()) {
-        decode_result(r, &it->second.res);
-        it->second.ready = true;
-        const double us = std::chrono::duration<double, std::micro>(now - it->second.t0).count();
-        it->second.res.latency_us = us;
-        it->second.res.queue_us =
-            std::chrono::duration<double, std::micro>(it->second.tAdmit - it->second.t0).count();
-        S->latUs.push_back(us);
+      std::lock_guard<std::mutex> lk(S->admitMu);
+      for (const Done &d : done) {
+        tickets.push_back(std::move(S->slotTicket[d.slot]));
+        S->freeSlots.push_back(d.slot);
       }
-      S->freeSlots.push_back(slot);
-      __atomic_fetch_sub(&S->inFlight, 1u, __ATOMIC_RELAXED);
-      S->nCompleted++;
+      S->inFlight.fetch_sub((uint32_t)done.size(), std::memory_order_relaxed);
       admit_locked(S);
+      if (S->waiting.empty() && S->inFlight.load() == 0) S->cvIdle.notify_all();
     }
-    S->cvDone.notify_all();
-    ++c;
+    for (size_t q = 0; q < done.size(); ++q) {
+      Ticket *t = tickets[q].get();
+      if (t == nullptr) continue;
+      td16_result res;
+      decode_result(done[q].r, &res);
+      const double us = std::chrono::duration<double, std::micro>(now - t->t0).count();
+      res.latency_us = us;
+      res.queue_us = std::chrono::duration<double, std::micro>(t->tAdmit - t->t0).count();
+      S->hist.add(us);
+      {
+        std::lock_guard<std::mutex> l(t->m);
+        t->res = res;
+        t->state.store(1, std::memory_order_release);
+      }
+      t->cv.notify_all();
+    }
+    S->nCompleted.fetch_add(done.size(), std::memory_order_relaxed);
   }
 }
 
@@ -549,12 +693,17 @@ int td16_shutdown(char *msg, int cap) {
     if (gBatch.stream) {
       hipSetDevice(gBatch.device);
       hipStreamSynchronize(gBatch.stream);
+      hipEventDestroy(gBatch.evUp);
       hipEventDestroy(gBatch.ev0);
       hipEventDestroy(gBatch.ev1);
       hipStreamDestroy(gBatch.stream);
     }
     gBatch.buf.release();
     gBatch = BatchCtx();
+  }
+  {
+    std::lock_guard<std::mutex> lk(gStreamMu);
+    gStream.release();
   }
   return TD16_OK;
 }
@@ -627,94 +776,22 @@ td16_batch_options normalize_batch_options(const td16_batch_options *opt) {
 
 }  // namespace
 
-// Host side of a batch: validates and relabels every request, answers the
-// trivial ones in `results`, and returns the rest in feed order (`where[q]`
-// is the caller's index of jobs[q]).
-int prepare_batch(const td16_request *reqs, int count, td16_result *results,
-                  std::vector<Job> &jobs, std::vector<int> &where, char *msg, int cap) {
-  jobs.clear();
-  where.clear();
-  jobs.reserve((size_t)count);
-  for (int q = 0; q < count; ++q) {
-    Job job;
-    int trivial = 0;
-    const int st = make_job_checked(&reqs[q], &job, &trivial, msg, cap);
-    if (st != TD16_OK) {
-      std::memset(&results[q], 0, sizeof(td16_result));
-      results[q].status = st;
-      return st;
-    }
-    if (trivial) {
-      trivial_result(trivial, &results[q]);
-      continue;
-    }
-    job.tag = (uint32_t)q;
-    jobs.push_back(job);
-    where.push_back(q);
-  }
-  const uint32_t nj = (uint32_t)jobs.size();
-  // Feed order: longest expected search first.  Heavy requests then start
-  // with the launch and are split once the feed drains, instead of starting
-  // late and becoming the launch's tail.  Long searches come from requests
-  // without slack: the first depth at which a witness is possible at all
-  // (every move gaining two odd cycles) is the move bound itself, so every
-  // path must be perfect; among those the deeper bound is the longer search
-  // (on the batched mix every request that needed 1,024 nodes or more had no
-  // slack, averaging 27,201 nodes at bound 8 against 341 at 7 and 22 at 6,
-  // and about 5 with slack).  Those go first, deepest first; the rest follow
-  // by move bound, then symbols.  The sorted list is dealt round-robin to the
-  // feed shards (contiguous ranges of the table, see init_feed), so every
-  // shard runs its own longest-first sequence.
-  {
-    std::vector<uint32_t> order(nj);
-    for (uint32_t q = 0; q < nj; ++q) order[q] = q;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-      const Job &x = jobs[a], &y = jobs[b];
-      const bool tx = (x.flags & kFlagTight) != 0, ty = (y.flags & kFlagTight) != 0;
-      if (tx != ty) return tx;
-      return x.maxDepth != y.maxDepth ? x.maxDepth > y.maxDepth : x.N > y.N;
-    });
-    const uint32_t per = (nj + kFeedShards - 1) / kFeedShards;
-    std::vector<Job> dealtJobs(nj);
-    std::vector<int> dealtWhere(nj);
-    uint32_t next = 0;
-    for (uint32_t t = 0; next < nj; ++t) {
-      for (uint32_t s = 0; s < (uint32_t)kFeedShards && next < nj; ++s) {
-        const uint64_t pos = (uint64_t)s * per + t;
-        const uint64_t end = std::min<uint64_t>((uint64_t)(s + 1) * per, nj);
-        if (pos >= end) continue;
-        dealtJobs[pos] = jobs[order[next]];
-        dealtWhere[pos] = where[order[next]];
-        ++next;
-      }
-    }
-    jobs.swap(dealtJobs);
-    where.swap(dealtWhere);
-  }
-  return TD16_OK;
-}
-
 namespace {
 
-// Device side of a batch: `o.repeats` persistent launches over the prepared
-// requests; results of the last one land in results[where[q]].
-int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
-              const td16_batch_options &o, td16_result *results, td16_batch_stats *stats,
-              char *msg, int cap) {
-  const uint32_t nj = (uint32_t)jobs.size();
-  td16_batch_stats localStats;
-  std::memset(&localStats, 0, sizeof(localStats));
-  std::lock_guard<std::mutex> lk(gBatchMu);
-  DeviceInfo dev;
+// Makes the process's batch context fit `capJobsNeeded` requests on device
+// o.device (the caller holds gBatchMu).
+int ensure_batch_ctx(const td16_batch_options &o, uint32_t capJobsNeeded, DeviceInfo &dev,
+                     char *msg, int cap) {
   int st = device_info(o.device, &dev, msg, cap);
   if (st != TD16_OK) return st;
   TD16_HIP(hipSetDevice(o.device));
-  if (gBatch.device != o.device || gBatch.buf.jobsCap < std::max<uint32_t>(nj, 1) ||
+  if (gBatch.device != o.device || gBatch.buf.jobsCap < std::max<uint32_t>(capJobsNeeded, 1) ||
       gBatch.queueBytes != o.queue_bytes) {
     reset_batch_graph();
     if (gBatch.stream) {
       hipSetDevice(gBatch.device);
       hipStreamSynchronize(gBatch.stream);
+      hipEventDestroy(gBatch.evUp);
       hipEventDestroy(gBatch.ev0);
       hipEventDestroy(gBatch.ev1);
       hipStreamDestroy(gBatch.stream);
@@ -725,24 +802,33 @@ int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
     gBatch.device = o.device;
     gBatch.queueBytes = o.queue_bytes;
     TD16_HIP(hipStreamCreateWithFlags(&gBatch.stream, hipStreamNonBlocking));
+    TD16_HIP(hipEventCreate(&gBatch.evUp));
     TD16_HIP(hipEventCreate(&gBatch.ev0));
     TD16_HIP(hipEventCreate(&gBatch.ev1));
     uint32_t capJobs = 1024;
-    while (capJobs < nj) capJobs *= 2;
+    while (capJobs < capJobsNeeded) capJobs *= 2;
     st = alloc_buffers(gBatch.buf, o.device, capJobs, o.queue_bytes, gBatch.stream, msg, cap);
     if (st != TD16_OK) return st;
   }
+  return TD16_OK;
+}
+
+// Device side of a batch: uploads the nj requests prepared in the pinned
+// table, runs `o.repeats` persistent launches over them and decodes the last
+// one's results into results[q] for the launched requests among the
+// caller's `count` (request q at feed position pos[q]; the caller holds
+// gBatchMu).
+int launch_prepared(uint32_t nj, const uint32_t *pos, uint32_t count,
+                    const td16_batch_options &o, const DeviceInfo &dev, td16_result *results,
+                    td16_batch_stats &localStats, char *msg, int cap) {
   Buffers &b = gBatch.buf;
   int blocks = 0;
   bool wide = false;
   batch_grid(o.waves_per_cu, dev.cus, dev.blocksPerCU, dev.blocksPerCUWide, &blocks, &wide);
   localStats.blocks = blocks;
   localStats.waves = blocks * kWavesPerBlock;
-  if (nj == 0) {
-    if (stats) *stats = localStats;
-    return TD16_OK;
-  }
-  std::memcpy(b.hJobs, jobs.data(), sizeof(Job) * nj);
+  if (nj == 0) return TD16_OK;
+  TD16_HIP(hipEventRecord(gBatch.evUp, gBatch.stream));
   TD16_HIP(hipMemcpyAsync(b.dJobs, b.hJobs, sizeof(Job) * nj, hipMemcpyHostToDevice, gBatch.stream));
   std::memset(b.hCtrl, 0, sizeof(Control));
   const size_t ctrlBytes = init_feed(b.hCtrl, nj);
@@ -781,17 +867,42 @@ int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where,
     TD16_HIP(hipEventElapsedTime(&ms, gBatch.ev0, gBatch.ev1));
     kernelMs += ms;
     wallMs += std::chrono::duration<double, std::milli>(w1 - w0).count();
+    if (rep == 0) {
+      float up = 0.f;  // the table upload and the first control copy
+      TD16_HIP(hipEventElapsedTime(&up, gBatch.evUp, gBatch.ev0));
+      localStats.upload_ms = up;
+    }
   }
   localStats.kernel_ms = kernelMs / o.repeats;
   localStats.wall_ms = wallMs / o.repeats;
-  for (uint32_t q = 0; q < nj; ++q) {
-    const DevResult &r = b.hResults[q];
-    td16_result &out = results[where[q]];
-    decode_result(r, &out);
-    localStats.expanded += r.expanded;
-    localStats.solutions += r.solutions;
-    localStats.donations += (uint64_t)r.split;
-  }
+  const auto d0 = std::chrono::steady_clock::now();
+  decode_batch(b.hResults, pos, count, results, o.host_threads, &localStats.expanded,
+               &localStats.solutions, &localStats.donations);
+  localStats.decode_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - d0).count();
+  return TD16_OK;
+}
+
+// `o.repeats` persistent launches over requests prepared once (the
+// autotuner's inner loop); results of the last one land in results[where[q]].
+int run_batch(const std::vector<Job> &jobs, const std::vector<int> &where, int count,
+              const td16_batch_options &o, td16_result *results, td16_batch_stats *stats,
+              char *msg, int cap) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t nj = (uint32_t)jobs.size();
+  td16_batch_stats localStats;
+  std::memset(&localStats, 0, sizeof(localStats));
+  std::lock_guard<std::mutex> lk(gBatchMu);
+  DeviceInfo dev;
+  int st = ensure_batch_ctx(o, nj, dev, msg, cap);
+  if (st != TD16_OK) return st;
+  if (nj) std::memcpy(gBatch.buf.hJobs, jobs.data(), sizeof(Job) * nj);
+  std::vector<uint32_t> pos;
+  positions_of(where.data(), nj, (uint32_t)count, pos);
+  st = launch_prepared(nj, pos.data(), (uint32_t)count, o, dev, results, localStats, msg, cap);
+  if (st != TD16_OK) return st;
+  localStats.total_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = localStats;
   return TD16_OK;
 }
@@ -805,12 +916,151 @@ int td16_batch(const td16_request *reqs, int count, td16_result *results,
                const td16_batch_options *opt, td16_batch_stats *stats, char *msg, int cap) {
   if (count < 0 || (count > 0 && (reqs == nullptr || results == nullptr)))
     return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid batch arguments");
+  const auto t0 = std::chrono::steady_clock::now();
   const td16_batch_options o = normalize_batch_options(opt);
-  std::vector<Job> jobs;
-  std::vector<int> where;
-  const int st = prepare_batch(reqs, count, results, jobs, where, msg, cap);
+  td16_batch_stats localStats;
+  std::memset(&localStats, 0, sizeof(localStats));
+  std::lock_guard<std::mutex> lk(gBatchMu);
+  DeviceInfo dev;
+  int st = ensure_batch_ctx(o, (uint32_t)count, dev, msg, cap);
   if (st != TD16_OK) return st;
-  return run_batch(jobs, where, o, results, stats, msg, cap);
+  // requests go straight to their feed positions in the pinned upload table
+  gBatch.where.resize((size_t)std::max(count, 1));
+  uint32_t nj = 0;
+  st = prepare_batch_into(reqs, count, results, gBatch.buf.hJobs, gBatch.where.data(), &nj,
+                          gBatch.ws, o.host_threads, msg, cap);
+  if (st != TD16_OK) return st;
+  localStats.prepare_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  st = launch_prepared(nj, gBatch.ws.pos.data(), (uint32_t)count, o, dev, results, localStats,
+                       msg, cap);
+  if (st != TD16_OK) return st;
+  localStats.total_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = localStats;
+  return TD16_OK;
+}
+
+int td16_batch_stream(const td16_request *reqs, int count, int chunk, td16_result *results,
+                      const td16_batch_options *opt, td16_batch_stats *stats, char *msg,
+                      int cap) {
+  if (count < 0 || (count > 0 && (reqs == nullptr || results == nullptr)))
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid batch arguments");
+  if (chunk < 1)
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "chunk must be at least 1 request");
+  const auto t0 = std::chrono::steady_clock::now();
+  const td16_batch_options o = normalize_batch_options(opt);
+  td16_batch_stats total;
+  std::memset(&total, 0, sizeof(total));
+  if (count == 0) {
+    if (stats) *stats = total;
+    return TD16_OK;
+  }
+  chunk = std::min(chunk, count);
+  std::lock_guard<std::mutex> lk(gStreamMu);
+  DeviceInfo dev;
+  int st = device_info(o.device, &dev, msg, cap);
+  if (st != TD16_OK) return st;
+  StreamCtx &X = gStream;
+  TD16_HIP(hipSetDevice(o.device));
+  if (X.device != o.device || X.buf[0].jobsCap < (uint32_t)chunk ||
+      X.queueBytes != o.queue_bytes) {
+    X.release();
+    X.device = o.device;
+    X.queueBytes = o.queue_bytes;
+    TD16_HIP(hipStreamCreateWithFlags(&X.run, hipStreamNonBlocking));
+    TD16_HIP(hipStreamCreateWithFlags(&X.copy, hipStreamNonBlocking));
+    uint32_t capJobs = 1024;
+    while (capJobs < (uint32_t)chunk) capJobs *= 2;
+    for (int s = 0; s < 2; ++s) {
+      for (hipEvent_t *e : {&X.up0[s], &X.up1[s], &X.k0[s], &X.k1[s]})
+        TD16_HIP(hipEventCreate(e));
+      st = alloc_buffers(X.buf[s], o.device, capJobs, o.queue_bytes, X.run, msg, cap);
+      if (st != TD16_OK) return st;
+    }
+  }
+  int blocks = 0;
+  bool wide = false;
+  batch_grid(o.waves_per_cu, dev.cus, dev.blocksPerCU, dev.blocksPerCUWide, &blocks, &wide);
+  total.blocks = blocks;
+  total.waves = blocks * kWavesPerBlock;
+  const int chunks = (count + chunk - 1) / chunk;
+  uint32_t nj[2] = {0, 0};
+  int base[2] = {0, 0}, cnt[2] = {0, 0};
+
+  // Chunk c: prepared on the host into context c % 2, uploaded on the copy
+  // stream, searched on the run stream once its upload is done.
+  auto enqueue = [&](int s) -> int {
+    Buffers &b = X.buf[s];
+    std::memset(b.hCtrl, 0, sizeof(Control));
+    const size_t ctrlBytes = init_feed(b.hCtrl, nj[s]);
+    TD16_HIP(hipEventRecord(X.up0[s], X.copy));
+    TD16_HIP(hipMemcpyAsync(b.dJobs, b.hJobs, sizeof(Job) * nj[s], hipMemcpyHostToDevice, X.copy));
+    TD16_HIP(hipMemcpyAsync(b.dCtrl, b.hCtrl, ctrlBytes, hipMemcpyHostToDevice, X.copy));
+    TD16_HIP(hipEventRecord(X.up1[s], X.copy));
+    TD16_HIP(hipStreamWaitEvent(X.run, X.up1[s], 0));
+    Params P = make_params(b, dev);
+    apply_batch_knobs(P, o);
+    TD16_HIP(hipEventRecord(X.k0[s], X.run));
+    TD16_HIP((hipError_t)launch_search(P, blocks, wide, X.run));
+    TD16_HIP(hipEventRecord(X.k1[s], X.run));
+    return TD16_OK;
+  };
+  auto finish = [&](int s) -> int {
+    if (nj[s] == 0) return TD16_OK;
+    const auto w0 = std::chrono::steady_clock::now();
+    TD16_HIP(hipEventSynchronize(X.k1[s]));
+    total.wall_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    float ms = 0.f, up = 0.f;
+    TD16_HIP(hipEventElapsedTime(&ms, X.k0[s], X.k1[s]));
+    TD16_HIP(hipEventElapsedTime(&up, X.up0[s], X.up1[s]));
+    total.kernel_ms += ms;
+    total.upload_ms += up;
+    const auto d0 = std::chrono::steady_clock::now();
+    uint64_t e = 0, so = 0, dn = 0;
+    decode_batch(X.buf[s].hResults, X.ws[s].pos.data(), (uint32_t)cnt[s], results + base[s],
+                 o.host_threads, &e, &so, &dn);
+    total.decode_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - d0).count();
+    total.expanded += e;
+    total.solutions += so;
+    total.donations += dn;
+    return TD16_OK;
+  };
+
+  for (int c = 0; c <= chunks; ++c) {
+    const int s = c & 1;
+    if (c < chunks) {
+      // context s last served chunk c - 2, finished below in step c - 1
+      base[s] = c * chunk;
+      const int n = std::min(chunk, count - base[s]);
+      cnt[s] = n;
+      X.where[s].resize((size_t)n);
+      const auto p0 = std::chrono::steady_clock::now();
+      st = prepare_batch_into(reqs + base[s], n, results + base[s], X.buf[s].hJobs,
+                              X.where[s].data(), &nj[s], X.ws[s], o.host_threads, msg, cap);
+      total.prepare_ms +=
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0)
+              .count();
+      if (st != TD16_OK) {
+        hipStreamSynchronize(X.run);
+        return st;
+      }
+      if (nj[s]) {
+        st = enqueue(s);
+        if (st != TD16_OK) return st;
+      }
+    }
+    if (c >= 1) {
+      st = finish(s ^ 1);  // chunk c - 1, searched while chunk c was prepared
+      if (st != TD16_OK) return st;
+    }
+  }
+  total.total_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = total;
+  return TD16_OK;
 }
 
 int td16_autotune(const td16_request *reqs, int count, td16_result *results,
@@ -866,11 +1116,11 @@ int td16_autotune(const td16_request *reqs, int count, td16_result *results,
     o.donate_max = dm;
     o.check_every = ck;
     td16_batch_stats bs;
-    int rc = run_batch(jobs, where, o, cur.data(), &bs, msg, cap);
+    int rc = run_batch(jobs, where, count, o, cur.data(), &bs, msg, cap);
     if (rc != TD16_OK) return rc;
     std::vector<double> t;
     for (int r = 0; r < repeats; ++r) {
-      rc = run_batch(jobs, where, o, cur.data(), &bs, msg, cap);
+      rc = run_batch(jobs, where, count, o, cur.data(), &bs, msg, cap);
       if (rc != TD16_OK) return rc;
       t.push_back(bs.kernel_ms);
     }
@@ -952,28 +1202,36 @@ int td16_serve_start(const td16_serve_options *opt, char *msg, int cap) {
     return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "slots must be in 1..65536");
   if (opt->queue_capacity < 0)
     return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "queue capacity must not be negative");
-  if (opt->slot_bytes <= 0)
-    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "slot budget must be positive");
+  if (opt->slot_bytes < (int64_t)(sizeof(Item) + sizeof(uint64_t)))
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT,
+                   "the slot budget must hold at least one work item (%zu bytes)",
+                   sizeof(Item) + sizeof(uint64_t));
+  if (opt->deadline_ms < 0)
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "deadline_ms must not be negative");
   std::lock_guard<std::mutex> lk(gServeMu);
-  if (gServer) return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "the served loop is already running");
+  if (current_server())
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "the served loop is already running");
   DeviceInfo dev;
   int st = device_info(opt->device, &dev, msg, cap);
   if (st != TD16_OK) return st;
   TD16_HIP(hipSetDevice(opt->device));
   size_t freeB = 0, totalB = 0;
   TD16_HIP(hipMemGetInfo(&freeB, &totalB));
-  // admission budget: every slot may hold slot_bytes of HBM for its work items
-  const unsigned long long need = (unsigned long long)opt->slot_bytes * (unsigned long long)opt->slots;
-  const unsigned long long queueBytes = std::min<unsigned long long>(need, 4ULL << 30);
+  // every slot holds up to slot_bytes of HBM for its request's queued
+  // subtrees (sorting_search.cu:755-793 sizes each slot's queues from its
+  // budget; slot_manager.cu:87-106 checks budget x slots against free memory)
+  const unsigned long long queueBytes =
+      (unsigned long long)opt->slot_bytes * (unsigned long long)opt->slots;
   if (queueBytes > freeB)
     return set_msg(msg, cap, TD16_OUT_OF_MEMORY,
                    "device %d has %zu free bytes, but the served loop requires %llu bytes",
                    opt->device, freeB, queueBytes);
-  auto S = std::make_unique<Server>();
+  auto S = std::make_shared<Server>();
   S->device = opt->device;
   S->dev = dev;
   S->slots = (uint32_t)opt->slots;
   S->queueCap = (uint32_t)opt->queue_capacity;
+  S->deadlineMs = opt->deadline_ms;
   uint32_t ring = 64;
   while (ring < 2 * S->slots) ring *= 2;
   S->ringCap = ring;
@@ -999,7 +1257,7 @@ int td16_serve_start(const td16_serve_options *opt, char *msg, int cap) {
   TD16_HIP(hipHostGetDevicePointer((void **)&S->dComp, S->hComp, 0));
   S->freeSlots.resize(S->slots);
   for (uint32_t q = 0; q < S->slots; ++q) S->freeSlots[q] = S->slots - 1 - q;
-  S->slotTicket.assign(S->slots, 0);
+  S->slotTicket.assign(S->slots, nullptr);
 
   std::memset(S->buf.hCtrl, 0, sizeof(Control));
   TD16_HIP(hipMemcpyAsync(S->buf.dCtrl, S->buf.hCtrl, sizeof(Control), hipMemcpyHostToDevice,
@@ -1015,6 +1273,9 @@ int td16_serve_start(const td16_serve_options *opt, char *msg, int cap) {
   // every 32 expansions keep the p99 lower (20k/s: 141-143 against 164-171
   // us at 16; the median is the same)
   P.checkEvery = 32;
+  // a request's queued subtrees stay within its slot's budget
+  const uint64_t perSlot = (uint64_t)opt->slot_bytes / (sizeof(Item) + sizeof(uint64_t));
+  P.itemCap = (int32_t)std::min<uint64_t>(perSlot, (uint64_t)INT32_MAX);
   P.ringCap = ring;
   P.host = S->dHost;
   P.hostJobs = S->dRing;
@@ -1024,123 +1285,183 @@ int td16_serve_start(const td16_serve_options *opt, char *msg, int cap) {
   const int wantWaves = opt->waves_per_cu > 0 ? opt->waves_per_cu : 4;
   const int perCU = std::max(1, std::min(dev.blocksPerCU, wantWaves / kWavesPerBlock));
   TD16_HIP((hipError_t)launch_search(P, dev.cus * perCU, false, S->stream));
-  S->running = true;
-  Server *raw = S.release();
-  raw->poller = std::thread(poll_loop, raw);
-  gServer = raw;
+  S->poller = std::thread(poll_loop, S.get());
+  std::lock_guard<std::mutex> pk(gServerPtrMu);
+  gServer = S;
   return TD16_OK;
 }
 
 namespace td16 {
 namespace {
-int serve_submit_at(const td16_request *req, int nonblocking, uint64_t *ticket,
-                    std::chrono::steady_clock::time_point arrival, char *msg, int cap);
-}
-}  // namespace td16
 
-int td16_serve_submit(const td16_request *req, int nonblocking, uint64_t *ticket, char *msg,
-                      int cap) {
-  return td16::serve_submit_at(req, nonblocking, ticket, std::chrono::steady_clock::now(), msg,
-                               cap);
-}
-
-namespace td16 {
-namespace {
-int serve_submit_at(const td16_request *req, int nonblocking, uint64_t *ticket,
-                    std::chrono::steady_clock::time_point arrival, char *msg, int cap) {
-  Server *S = gServer;
-  if (S == nullptr) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
+int serve_submit_at(const std::shared_ptr<Server> &S, const td16_request *req, int nonblocking,
+                    uint64_t *ticket, std::chrono::steady_clock::time_point arrival, char *msg,
+                    int cap) {
   if (req == nullptr || ticket == nullptr)
     return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "request and ticket must not be null");
+  if (S->failed.load())
+    return set_msg(msg, cap, TD16_DEVICE_ERROR, "served loop: %s", S->failure);
   Job job;
   int trivial = 0;
   const int st = make_job_checked(req, &job, &trivial, msg, cap);
   if (st != TD16_OK) return st;
-  std::unique_lock<std::mutex> lk(S->mu);
-  const uint64_t t = S->nextTicket++;
-  Ticket &tk = S->tickets[t];
-  tk.t0 = arrival;
-  *ticket = t;
-  S->nSubmitted++;
+  auto tk = std::make_shared<Ticket>();
+  tk->t0 = arrival;
+  const uint64_t t = S->nextTicket.fetch_add(1, std::memory_order_relaxed);
   if (trivial) {
-    trivial_result(trivial, &tk.res);
-    tk.ready = true;
-    S->nCompleted++;
-    S->latUs.push_back(0.0);
+    trivial_result(trivial, &tk->res);
+    tk->state.store(1, std::memory_order_release);
+    {
+      TicketShard &sh = S->shard(t);
+      std::lock_guard<std::mutex> l(sh.mu);
+      sh.map.emplace(t, tk);
+    }
+    *ticket = t;
+    S->nSubmitted.fetch_add(1, std::memory_order_relaxed);
+    S->nCompleted.fetch_add(1, std::memory_order_relaxed);
+    S->hist.add(0.0);
     return TD16_OK;
   }
-  // GPUSortingSearch.java:46-52: pending capacity = slots + queue capacity
-  const size_t limit = (size_t)S->queueCap + S->slots;
-  if (S->waiting.size() >= limit) {
-    if (nonblocking) {
-      S->tickets.erase(t);
-      S->nSubmitted--;
-      S->nRejected++;
-      return set_msg(msg, cap, TD16_QUEUE_FULL, "the served loop's pending queue is full");
-    }
-    S->cvSpace.wait(lk, [&] { return S->waiting.size() < limit || S->stopping; });
+  {
+    TicketShard &sh = S->shard(t);
+    std::lock_guard<std::mutex> l(sh.mu);
+    sh.map.emplace(t, tk);
   }
-  S->waiting.emplace_back(t, job);
-  admit_locked(S);
+  auto drop = [&](int status, const char *why) {
+    TicketShard &sh = S->shard(t);
+    std::lock_guard<std::mutex> l(sh.mu);
+    sh.map.erase(t);
+    return set_msg(msg, cap, status, "%s", why);
+  };
+  {
+    std::unique_lock<std::mutex> lk(S->admitMu);
+    // GPUSortingSearch.java:46-52: pending capacity = slots + queue capacity
+    const size_t limit = (size_t)S->queueCap + S->slots;
+    if (S->stopping) {
+      lk.unlock();
+      return drop(TD16_NOT_INITIALIZED, "the served loop is stopping");
+    }
+    if (S->waiting.size() >= limit) {
+      if (nonblocking) {
+        S->nRejected.fetch_add(1, std::memory_order_relaxed);
+        lk.unlock();
+        return drop(TD16_QUEUE_FULL, "the served loop's pending queue is full");
+      }
+      S->cvSpace.wait(lk, [&] {
+        return S->waiting.size() < limit || S->stopping || S->failed.load();
+      });
+      if (S->stopping || S->failed.load()) {
+        lk.unlock();
+        return drop(S->stopping ? TD16_NOT_INITIALIZED : TD16_DEVICE_ERROR,
+                    S->stopping ? "the served loop is stopping" : "the served loop failed");
+      }
+    }
+    S->waiting.emplace_back(tk, job);
+    admit_locked(S.get());
+  }
+  *ticket = t;
+  S->nSubmitted.fetch_add(1, std::memory_order_relaxed);
   return TD16_OK;
 }
+
+int serve_wait(const std::shared_ptr<Server> &S, uint64_t ticket, td16_result *res, char *msg,
+               int cap) {
+  TicketPtr tk;
+  {
+    TicketShard &sh = S->shard(ticket);
+    std::lock_guard<std::mutex> l(sh.mu);
+    auto it = sh.map.find(ticket);
+    if (it == sh.map.end()) return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "unknown ticket");
+    tk = it->second;
+  }
+  // most results arrive within tens of microseconds: spin a little first
+  const auto start = std::chrono::steady_clock::now();
+  for (int q = 0; q < 2000 && tk->state.load(std::memory_order_acquire) == 0; ++q) {
+#if defined(__x86_64__)
+    __builtin_ia32_pause();
+#endif
+  }
+  {
+    std::unique_lock<std::mutex> l(tk->m);
+    const int slice = S->deadlineMs > 0 ? std::min(S->deadlineMs, 20) : 20;
+    while (tk->state.load(std::memory_order_acquire) == 0) {
+      tk->cv.wait_for(l, std::chrono::milliseconds(slice));
+      if (tk->state.load(std::memory_order_acquire) != 0) break;
+      if (S->failed.load())
+        return set_msg(msg, cap, TD16_DEVICE_ERROR, "served loop: %s", S->failure);
+      if (S->deadlineMs > 0 &&
+          std::chrono::steady_clock::now() - start > std::chrono::milliseconds(S->deadlineMs))
+        // the ticket stays valid: a later wait may still collect the result
+        return set_msg(msg, cap, TD16_DEVICE_ERROR,
+                       "served loop: no result within the %d ms deadline", S->deadlineMs);
+    }
+  }
+  if (tk->state.load(std::memory_order_acquire) == 2) {
+    TicketShard &sh = S->shard(ticket);
+    std::lock_guard<std::mutex> l(sh.mu);
+    sh.map.erase(ticket);
+    return set_msg(msg, cap, TD16_DEVICE_ERROR, "served loop: %s", S->failure);
+  }
+  *res = tk->res;
+  TicketShard &sh = S->shard(ticket);
+  std::lock_guard<std::mutex> l(sh.mu);
+  sh.map.erase(ticket);
+  return TD16_OK;
+}
+
 }  // namespace
 }  // namespace td16
 
+int td16_serve_submit(const td16_request *req, int nonblocking, uint64_t *ticket, char *msg,
+                      int cap) {
+  auto S = current_server();
+  if (!S) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
+  return serve_submit_at(S, req, nonblocking, ticket, std::chrono::steady_clock::now(), msg, cap);
+}
+
 int td16_serve_wait(uint64_t ticket, td16_result *res, char *msg, int cap) {
-  Server *S = gServer;
-  if (S == nullptr) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
-  std::unique_lock<std::mutex> lk(S->mu);
-  auto it = S->tickets.find(ticket);
-  if (it == S->tickets.end()) return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "unknown ticket");
-  S->cvDone.wait(lk, [&] { return it->second.ready; });
-  *res = it->second.res;
-  S->tickets.erase(it);
-  return TD16_OK;
+  if (res == nullptr) return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "result must not be null");
+  auto S = current_server();
+  if (!S) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
+  return serve_wait(S, ticket, res, msg, cap);
 }
 
 int td16_serve_search(const td16_request *req, td16_result *res, char *msg, int cap) {
+  auto S = current_server();
+  if (!S) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
   uint64_t t = 0;
-  const int st = td16_serve_submit(req, 0, &t, msg, cap);
+  const int st = serve_submit_at(S, req, 0, &t, std::chrono::steady_clock::now(), msg, cap);
   if (st != TD16_OK) return st;
-  return td16_serve_wait(t, res, msg, cap);
+  return serve_wait(S, t, res, msg, cap);
 }
 
 int td16_serve_stats_get(td16_serve_stats *out, int resetLatency, char *msg, int cap) {
-  Server *S = gServer;
-  if (S == nullptr) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
-  std::lock_guard<std::mutex> lk(S->mu);
+  if (out == nullptr) return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "stats must not be null");
+  auto S = current_server();
+  if (!S) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
   std::memset(out, 0, sizeof(*out));
-  out->submitted = S->nSubmitted;
-  out->completed = S->nCompleted;
-  out->rejected = S->nRejected;
-  out->in_flight = (int32_t)S->inFlight;
-  out->waiting = (int32_t)S->waiting.size();
-  if (!S->latUs.empty()) {
-    std::vector<double> v = S->latUs;
-    std::sort(v.begin(), v.end());
-    auto pct = [&](double p) {
-      size_t i = (size_t)(p * (double)(v.size() - 1) + 0.5);
-      return v[std::min(i, v.size() - 1)];
-    };
-    out->p50_us = pct(0.50);
-    out->p99_us = pct(0.99);
-    double s = 0;
-    for (double x : v) s += x;
-    out->mean_us = s / (double)v.size();
+  out->submitted = S->nSubmitted.load();
+  out->completed = S->nCompleted.load();
+  out->rejected = S->nRejected.load();
+  out->in_flight = (int32_t)S->inFlight.load();
+  {
+    std::lock_guard<std::mutex> lk(S->admitMu);
+    out->waiting = (int32_t)S->waiting.size();
   }
-  if (resetLatency) S->latUs.clear();
+  out->p50_us = S->hist.quantile(0.50);
+  out->p99_us = S->hist.quantile(0.99);
+  out->mean_us = S->hist.mean();
+  if (resetLatency) S->hist.reset();
   return TD16_OK;
 }
 
 int td16_serve_replay(const td16_request *reqs, const double *arrivalUs, int count,
                       td16_result *results, double *elapsedUs, char *msg, int cap) {
-  if (gServer == nullptr)
-    return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
+  auto S = current_server();
+  if (!S) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
   if (count < 0 || (count > 0 && (reqs == nullptr || arrivalUs == nullptr || results == nullptr)))
     return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid replay arguments");
-  td16_serve_stats tmp;
-  td16_serve_stats_get(&tmp, 1, nullptr, 0);  // reset the latency window
+  S->hist.reset();
   std::vector<uint64_t> tickets((size_t)count);
   const auto start = std::chrono::steady_clock::now();
   for (int q = 0; q < count; ++q) {
@@ -1150,11 +1471,11 @@ int td16_serve_replay(const td16_request *reqs, const double *arrivalUs, int cou
       __builtin_ia32_pause();
 #endif
     }
-    const int st = td16::serve_submit_at(&reqs[q], 0, &tickets[(size_t)q], due, msg, cap);
+    const int st = serve_submit_at(S, &reqs[q], 0, &tickets[(size_t)q], due, msg, cap);
     if (st != TD16_OK) return st;
   }
   for (int q = 0; q < count; ++q) {
-    const int st = td16_serve_wait(tickets[(size_t)q], &results[q], msg, cap);
+    const int st = serve_wait(S, tickets[(size_t)q], &results[q], msg, cap);
     if (st != TD16_OK) return st;
   }
   if (elapsedUs)
@@ -1163,32 +1484,116 @@ int td16_serve_replay(const td16_request *reqs, const double *arrivalUs, int cou
   return TD16_OK;
 }
 
+int td16_serve_closed_loop(const td16_request *reqs, int count, int producers,
+                           td16_result *results, double *elapsedUs, char *msg, int cap) {
+  auto S = current_server();
+  if (!S) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
+  if (count < 0 || (count > 0 && (reqs == nullptr || results == nullptr)))
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid closed-loop arguments");
+  if (producers < 1 || producers > 1024)
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "producers must be in 1..1024");
+  S->hist.reset();
+  std::atomic<int> firstErr{TD16_OK};
+  std::mutex errMu;
+  std::string err;
+  const auto start = std::chrono::steady_clock::now();
+  std::vector<std::thread> ths;
+  for (int p = 0; p < producers; ++p)
+    ths.emplace_back([&, p] {
+      char m[256];
+      for (int q = p; q < count; q += producers) {
+        if (firstErr.load(std::memory_order_relaxed) != TD16_OK) return;
+        uint64_t t = 0;
+        int st = serve_submit_at(S, &reqs[q], 0, &t, std::chrono::steady_clock::now(), m,
+                                 sizeof m);
+        if (st == TD16_OK) st = serve_wait(S, t, &results[q], m, sizeof m);
+        if (st != TD16_OK) {
+          int expected = TD16_OK;
+          if (firstErr.compare_exchange_strong(expected, st)) {
+            std::lock_guard<std::mutex> l(errMu);
+            err = m;
+          }
+          return;
+        }
+      }
+    });
+  for (auto &t : ths) t.join();
+  if (elapsedUs)
+    *elapsedUs = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - start)
+                     .count();
+  if (firstErr.load() != TD16_OK) return set_msg(msg, cap, firstErr.load(), "%s", err.c_str());
+  return TD16_OK;
+}
+
 int td16_serve_stop(char *msg, int cap) {
   std::lock_guard<std::mutex> lk(gServeMu);
-  Server *S = gServer;
-  if (S == nullptr) return TD16_OK;
+  auto S = current_server();
+  if (!S) return TD16_OK;
+  auto since = [](std::chrono::steady_clock::time_point t0, int ms) {
+    return ms > 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(ms);
+  };
+  // no new request is admitted from here on and blocked submitters give up;
+  // admitted and waiting requests finish (within the deadline, if any)
+  const auto t0 = std::chrono::steady_clock::now();
+  bool drained = true;
   {
-    std::unique_lock<std::mutex> l2(S->mu);
+    std::unique_lock<std::mutex> l2(S->admitMu);
     S->stopping = true;
     S->cvSpace.notify_all();
-    S->cvDone.wait(l2, [&] { return S->waiting.empty() && S->inFlight == 0; });
+    while (!(S->waiting.empty() && S->inFlight.load() == 0)) {
+      if (S->failed.load() || since(t0, S->deadlineMs)) {
+        drained = false;
+        break;
+      }
+      S->cvIdle.wait_for(l2, std::chrono::milliseconds(20));
+    }
+    // the kernel exits once every request it took has finished, so the
+    // stop request is safe to send in any case
+    __atomic_store_n(&S->hHost->submitted, S->submittedSeq | kServeStop, __ATOMIC_RELEASE);
   }
-  __atomic_store_n(&S->hHost->submitted, S->submittedSeq | kServeStop, __ATOMIC_RELEASE);
+  // requests that can no longer be delivered end with an error, not a hang
+  if (!drained) fail_all(S.get(), "the served loop stopped before the request finished");
   hipSetDevice(S->device);
-  hipError_t e = hipStreamSynchronize(S->stream);
+  // the kernel exits within one idle sleep of seeing the stop request; with
+  // a deadline, wait at least a second for that
+  const auto t1 = std::chrono::steady_clock::now();
+  const int exitMs = S->deadlineMs > 0 ? std::max(S->deadlineMs, 1000) : 0;
+  hipError_t e;
+  while ((e = hipStreamQuery(S->stream)) == hipErrorNotReady && !since(t1, exitMs))
+    std::this_thread::sleep_for(std::chrono::microseconds(50));
   S->pollerExit.store(true, std::memory_order_release);
-  S->poller.join();
+  S->cvWork.notify_all();
+  if (S->poller.joinable()) S->poller.join();
+  {
+    std::lock_guard<std::mutex> pk(gServerPtrMu);
+    gServer.reset();
+  }
+  if (!drained || e != hipSuccess) {
+    // a kernel that may still run keeps its memory (it could write it yet)
+    if (e == hipErrorNotReady)
+      return set_msg(msg, cap, TD16_DEVICE_ERROR,
+                     "served loop: the kernel did not stop within %d ms", exitMs);
+    if (e != hipSuccess)
+      return set_msg(msg, cap, TD16_DEVICE_ERROR, "served kernel failed: %s", hipGetErrorName(e));
+    if (S->failed.load() && std::strcmp(S->failure, "the served loop stopped before the request "
+                                                    "finished") != 0)
+      return set_msg(msg, cap, TD16_DEVICE_ERROR, "served loop: %s", S->failure);
+    return set_msg(msg, cap, TD16_DEVICE_ERROR,
+                   "served loop: requests still running at the %d ms deadline were failed",
+                   S->deadlineMs);
+  }
   S->buf.release();
   hipHostFree(S->hHost);
   hipHostFree(S->hRing);
   hipHostFree(S->hRes);
   hipHostFree(S->hComp);
   hipStreamDestroy(S->stream);
-  gServer = nullptr;
-  delete S;
-  if (e != hipSuccess)
-    return set_msg(msg, cap, TD16_DEVICE_ERROR, "served kernel failed: %s", hipGetErrorName(e));
+  S->hHost = nullptr;
+  S->hRing = nullptr;
+  S->hRes = nullptr;
+  S->hComp = nullptr;
   return TD16_OK;
 }
 
 }  // extern "C"
+

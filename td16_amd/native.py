@@ -122,15 +122,40 @@ class BatchRun:
 
 def batch(reqs: Sequence[Request], device: int = 0, waves_per_cu: int = 0, split_depth: int = 0,
           donate_max: int = 0, use_graph: bool = False, repeats: int = 1,
-          queue_bytes: int = 0, check_every: int = 0) -> BatchRun:
+          queue_bytes: int = 0, check_every: int = 0, host_threads: int = 0,
+          out: Optional[ctypes.Array] = None) -> BatchRun:
+    """One persistent launch over ``reqs`` (``td16_batch``).  ``out``: a
+    result array of at least len(reqs) entries to fill (a new one
+    otherwise)."""
     n = len(reqs)
     arr = (Request * max(1, n))(*reqs) if not isinstance(reqs, ctypes.Array) else reqs
-    out = (Result * max(1, n))()
+    if out is None or len(out) < n:
+        out = (Result * max(1, n))()
     opt = BatchOptions(device, waves_per_cu, split_depth, donate_max, 1 if use_graph else 0,
-                       repeats, queue_bytes, check_every, 0)
+                       repeats, queue_bytes, check_every, host_threads)
     st = BatchStats()
     m = msgbuf()
     check(lib().td16_batch(arr, n, out, ctypes.byref(opt), ctypes.byref(st), m, len(m)), m)
+    return BatchRun(out, st)
+
+
+def batch_stream(reqs: Sequence[Request], chunk: int, device: int = 0, waves_per_cu: int = 0,
+                 split_depth: int = 0, donate_max: int = 0, queue_bytes: int = 0,
+                 check_every: int = 0, host_threads: int = 0,
+                 out: Optional[ctypes.Array] = None) -> BatchRun:
+    """Consecutive launches of at most ``chunk`` requests with the host's
+    preparation and decoding overlapped with the searches
+    (``td16_batch_stream``); ``out`` as in :func:`batch`."""
+    n = len(reqs)
+    arr = (Request * max(1, n))(*reqs) if not isinstance(reqs, ctypes.Array) else reqs
+    if out is None or len(out) < n:
+        out = (Result * max(1, n))()
+    opt = BatchOptions(device, waves_per_cu, split_depth, donate_max, 0, 1, queue_bytes,
+                       check_every, host_threads)
+    st = BatchStats()
+    m = msgbuf()
+    check(lib().td16_batch_stream(arr, n, chunk, out, ctypes.byref(opt), ctypes.byref(st), m,
+                                  len(m)), m)
     return BatchRun(out, st)
 
 
@@ -188,8 +213,8 @@ def autotune(reqs: Sequence[Request], device: int = 0, repeats: int = 3, use_gra
 # ---------------------------------------------------------------------------
 
 def serve_start(device: int = 0, slots: int = 64, queue_capacity: int = 512,
-                slot_bytes: int = 1 << 30, waves_per_cu: int = 0) -> None:
-    opt = ServeOptions(device, slots, queue_capacity, waves_per_cu, slot_bytes)
+                slot_bytes: int = 1 << 30, waves_per_cu: int = 0, deadline_ms: int = 0) -> None:
+    opt = ServeOptions(device, slots, queue_capacity, waves_per_cu, slot_bytes, deadline_ms, 0)
     m = msgbuf()
     check(lib().td16_serve_start(ctypes.byref(opt), m, len(m)), m)
 
@@ -233,6 +258,18 @@ def serve_replay(reqs: Sequence[Request], arrival_us: Sequence[float]):
     el = ctypes.c_double(0)
     m = msgbuf()
     check(lib().td16_serve_replay(arr, arrivals, n, out, ctypes.byref(el), m, len(m)), m)
+    return out, el.value, serve_stats()
+
+
+def serve_closed_loop(reqs: Sequence[Request], producers: int):
+    """Closed-loop load: `producers` threads each submit and wait, with no
+    pacing.  Returns (results, elapsed_us, stats)."""
+    n = len(reqs)
+    arr = (Request * max(1, n))(*reqs) if not isinstance(reqs, ctypes.Array) else reqs
+    out = (Result * max(1, n))()
+    el = ctypes.c_double(0)
+    m = msgbuf()
+    check(lib().td16_serve_closed_loop(arr, n, producers, out, ctypes.byref(el), m, len(m)), m)
     return out, el.value, serve_stats()
 
 

@@ -273,3 +273,54 @@ def test_autotune_picks_a_timed_point_and_keeps_answers(O, native):
     run = native.batch(reqs, **t.best)
     for q, req in enumerate(reqs):
         check_against_oracle(O, native, req, run.results[q])
+
+
+def mixed_batch(native, seed, count):
+    """Batched-mix products in both orders, with n = 10 exact distances, a
+    sorted input and two-symbol requests mixed in."""
+    ns, ks, perms = native.gen_products(seed, 12, 16, 4, 9, count)
+    reqs = [native.linear_request(int(perms[q]), int(ns[q]), int(ks[q]), lex_order=(q % 4 == 0))
+            for q in range(count)]
+    for p in native.gen_uniform(seed, 10, count // 10):
+        reqs.append(native.linear_request(int(p), 10, 9, native.MOVES_REF, native.FIRST, True))
+    reqs.append(native.make_request(list(range(6)), list(range(6)), 6, 0.0, 5, iddfs=True))
+    reqs.append(native.make_request([0, 1], [1, 0], 0, 1.0, 1))
+    return reqs
+
+
+def same_answers(native, reqs, a, b):
+    both_unsplit = 0
+    for q, req in enumerate(reqs):
+        x, y = a[q], b[q]
+        assert (x.status, x.found) == (y.status, y.found), q
+        if req.iddfs:
+            assert x.depth == y.depth, q
+        if not x.split and not y.split:
+            assert x.witness() == y.witness() and x.expanded == y.expanded, q
+            both_unsplit += 1
+    return both_unsplit
+
+
+@pytest.mark.parametrize("chunk", [1000, 7777])
+def test_stream_matches_single_batches(O, native, chunk):
+    """td16_batch_stream (double-buffered chunks) gives td16_batch's answers:
+    chunk sizes that do and do not divide the stream, trivial requests in
+    any chunk."""
+    reqs = mixed_batch(native, 41, 20000)
+    ref = native.batch(reqs)
+    got = native.batch_stream(reqs, chunk)
+    assert same_answers(native, reqs, ref.results, got.results) > len(reqs) // 2
+    for q in range(0, len(reqs), 97):
+        check_against_oracle(O, native, reqs[q], got.results[q])
+    st = got.stats
+    assert st.total_ms > 0 and st.kernel_ms > 0 and st.expanded > 0
+
+
+def test_prepare_thread_counts_give_the_same_batch(native):
+    """The host preparation is deterministic: one thread or sixteen, the
+    launch sees the same requests in the same feed order."""
+    reqs = mixed_batch(native, 43, 30000)
+    a = native.batch(reqs, host_threads=1)
+    b = native.batch(reqs, host_threads=16)
+    assert same_answers(native, reqs, a.results, b.results) > len(reqs) // 2
+    assert a.stats.prepare_ms > 0 and b.stats.decode_ms > 0

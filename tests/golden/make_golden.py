@@ -7,11 +7,14 @@ Outputs (all small JSON):
                      generators, nibble-packed uint64 little-endian, plus the
                      n and k bytes where they vary), and the first instances
                      of each set in clear
-  expected.json   -- oracle results on subsets: IDDFS distances (n = 10),
-                     first witnesses and node counts (mixes), the depth-8
-                     non-existence proofs at n = 16 with their exact node
-                     counts, COUNT_ALL counts, and the reference's own
-                     known-answer pairs.
+  expected.json   -- oracle results: the exact distance of all 10,000 n = 10
+                     permutations in both move sets, first witnesses and node
+                     counts (mixes), found / not-found of all 2,000 n = 16
+                     instances at depth 8, the depth-8 non-existence proofs
+                     with their exact node counts, exhaustive depth-8 counts
+                     of the zero-slack n = 16 instances whose trees the CPU
+                     finishes, COUNT_ALL counts of small products, and the
+                     reference's own known-answer pairs.
 The reference code is never imported or run: its fixtures enter as literals
 copied from its test files and logs (cited next to each).
 """
@@ -29,6 +32,10 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from oracle import oracle as O  # noqa: E402
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
+# exhaustive trees larger than this are left to size-independent checks
+ZERO_SLACK_CAP = 60_000_000
 
 
 def sha(*arrays) -> str:
@@ -98,6 +105,13 @@ def main():
         d_ref, _ = O.iddfs(beta, omega, O.MOVES_REF, 9)
         rows.append([int(c1[q]), d_all, d_ref])
     exp["config1_iddfs"] = rows
+    # config 1 in full: one digit per permutation, both move sets
+    ten = np.full(10000, 10, dtype=np.uint8)
+    nine = np.full(10000, 9, dtype=np.uint8)
+    for name, ms in (("all", O.MOVES_ALL), ("ref", O.MOVES_REF)):
+        res = O.search_batch_ex(ten, c1, nine, ms, O.FIRST, threads=THREADS, iddfs=True)
+        assert all(r.found == 1 for r in res)
+        exp.setdefault("config1_distances", {})[name] = "".join(str(r.depth) for r in res)
     # config 2: first 600 FIRST-mode REF searches
     rows = []
     for q in range(600):
@@ -120,6 +134,27 @@ def main():
         assert r.found == 0
         proofs.append({"q": q, "perm": int(p3full[q]), **result_dict(r, False)})
     exp["config3_nonexistence"] = proofs
+    # config 3 in full: found (1) / not found (0) within 8 moves, REF, FIRST
+    sixteen = np.full(2000, 16, dtype=np.uint8)
+    eight = np.full(2000, 8, dtype=np.uint8)
+    res = O.search_batch_ex(sixteen, p3full, eight, O.MOVES_REF, O.FIRST, threads=THREADS,
+                            greedy=True)
+    assert all(r.found in (0, 1) for r in res)
+    exp["config3_found"] = "".join(str(r.found) for r in res)
+    assert [q for q, r in enumerate(res) if r.found == 0] == [p["q"] for p in proofs]
+    # exhaustive depth-8 counts (COUNT_ALL) of the zero-slack instances: the
+    # odd-cycle bound of the root is 8, so only moves that gain two odd
+    # cycles stay reachable and the tree is the same under ALL and REF moves;
+    # the instances whose trees stay within ZERO_SLACK_CAP nodes
+    zero = [q for q in range(2000)
+            if (17 - O.count_odd_cycles(O.linear_to_pair(p3full[q], 16)[1])) // 2 == 8]
+    res = O.search_batch_ex(sixteen[:len(zero)], p3full[zero], eight[:len(zero)], O.MOVES_REF,
+                            O.COUNT_ALL, threads=THREADS, cap=ZERO_SLACK_CAP)
+    exp["config3_count_all"] = {
+        "cap": ZERO_SLACK_CAP, "zero_slack": len(zero),
+        "rows": [{"q": q, "perm": int(p3full[q]), "expanded": int(r.expanded),
+                  "solutions": int(r.solutions)}
+                 for q, r in zip(zero, res) if r.found != -2]}
     # COUNT_ALL: zero- and one-slack bounds on small products, both move sets
     rows = []
     ns5, ks5, p5 = O.gen_products(5, 8, 11, 4, 6, 40)

@@ -203,26 +203,99 @@ def test_dispatcher_smoke_like_reference():
         GPUSortingSearch.reset()
 
 
-def test_config1_full_size_properties(O, native):
+def replays(O, native, req, res):
+    beta, omega = pair(req)
+    sym = native.positions_to_symbols(beta, res.witness())
+    return O.validate(beta, omega, sym, req.max_moves) == 0
+
+
+def test_config1_full_size_distances(O, native, exp):
     """All 10,000 uniform n = 10 permutations (seed 0), exact distance by
-    iterative deepening in both move sets, checked by what holds at any size:
-    every witness replays to the identity, the distance lies between the
-    odd-cycle bound (N - odd) / 2 and the diameter d(10) = 6, and the
-    reference filter never beats the full move set."""
+    iterative deepening in both move sets, in one batched launch each: every
+    distance equals the oracle's committed one, every witness replays."""
     perms = native.gen_uniform(0, 10, 10000)
-    depth = {}
-    for ms in (native.MOVES_ALL, native.MOVES_REF):
+    for ms, key in ((native.MOVES_ALL, "all"), (native.MOVES_REF, "ref")):
+        want = exp["config1_distances"][key]
         reqs = [native.linear_request(int(p), 10, 9, ms, native.FIRST, True) for p in perms]
         run = native.batch(reqs)
-        d = []
         for q, req in enumerate(reqs):
             r = run.results[q]
-            assert r.found == 1, q
-            lb = (req.n - req.initial_even) // 2
-            assert lb <= r.depth <= 6, (q, lb, r.depth)
-            beta, omega = pair(req)
-            sym = native.positions_to_symbols(beta, r.witness())
-            assert O.validate(beta, omega, sym, req.max_moves) == 0, q
-            d.append(r.depth)
-        depth[ms] = d
-    assert all(a <= b for a, b in zip(depth[native.MOVES_ALL], depth[native.MOVES_REF]))
+            assert r.found == 1 and r.depth == int(want[q]), (key, q, r.depth, want[q])
+            assert replays(O, native, req, r), (key, q)
+
+
+def test_config1_plain_one_instance_per_launch(O, native, exp):
+    """Configuration 1 as specified, the plain method: each of the 10,000
+    permutations in its own launch through td16_search_ex (iterative
+    deepening over all 165 triples, and again under the reference filter),
+    seven slots each driven by its own thread as the reference's dispatcher
+    drives them.  Distances equal the committed oracle distances."""
+    perms = native.gen_uniform(0, 10, 10000)
+    slots = 7
+    native.init_slots(slots, [0], 64 << 20)
+    try:
+        for ms, key in ((native.MOVES_ALL, "all"), (native.MOVES_REF, "ref")):
+            want = exp["config1_distances"][key]
+            reqs = [native.linear_request(int(p), 10, 9, ms, native.FIRST, True) for p in perms]
+            out = [None] * len(reqs)
+            errors = []
+
+            def worker(s):
+                try:
+                    for q in range(s, len(reqs), slots):
+                        out[q] = native.search_ex(s, reqs[q])
+                except Exception as e:  # surfaced below
+                    errors.append(e)
+
+            ths = [threading.Thread(target=worker, args=(s,)) for s in range(slots)]
+            for t in ths:
+                t.start()
+            for t in ths:
+                t.join(120)
+            assert not errors, errors[:1]
+            for q, req in enumerate(reqs):
+                r = out[q]
+                assert r.found == 1 and r.depth == int(want[q]), (key, q)
+                if q % 10 == 0:
+                    assert replays(O, native, req, r), (key, q)
+    finally:
+        native.shutdown()
+
+
+def test_config3_every_instance_found_or_proved(O, native, exp):
+    """All 2,000 n = 16 instances (1,999 products of 9 transpositions and the
+    reverse permutation) at move bound 8 under the reference filter, in the
+    kernel's default order: found / not-found equals the oracle's for every
+    instance, every witness replays, and the non-existence proofs (complete
+    trees) have the oracle's node counts exactly."""
+    with open(os.path.join(GOLDEN, "instances.json")) as f:
+        inst = json.load(f)
+    _, _, p3 = native.gen_products(2, 16, 16, 9, 9, 1999)
+    perms = list(p3) + [inst["config3"]["reverse"]]
+    reqs = [native.linear_request(int(p), 16, 8) for p in perms]
+    run = native.batch(reqs)
+    want = exp["config3_found"]
+    proofs = {p["q"]: p for p in exp["config3_nonexistence"]}
+    for q, req in enumerate(reqs):
+        r = run.results[q]
+        assert r.found == int(want[q]), q
+        if r.found:
+            assert replays(O, native, req, r), q
+        else:
+            assert (r.expanded, r.solutions) == (proofs[q]["expanded"], 0), q
+
+
+def test_config3_exhaustive_counts(native, exp):
+    """Exhaustive depth-8 searches (COUNT_ALL) of the zero-slack n = 16
+    instances whose trees the CPU finishes: node and solution counts equal
+    the oracle's bit for bit, under the full move set and under the
+    reference filter (zero slack leaves only moves that gain two odd
+    cycles, so both trees are the same)."""
+    rows = exp["config3_count_all"]["rows"]
+    assert len(rows) >= 2
+    for ms in (native.MOVES_ALL, native.MOVES_REF):
+        reqs = [native.linear_request(r["perm"], 16, 8, ms, native.COUNT_ALL) for r in rows]
+        run = native.batch(reqs, waves_per_cu=20)
+        for q, row in enumerate(rows):
+            r = run.results[q]
+            assert (r.expanded, r.solutions) == (row["expanded"], row["solutions"]), (ms, row["q"])

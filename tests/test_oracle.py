@@ -372,3 +372,48 @@ def test_greedy_order_is_order_independent_where_it_must_be(O):
         cg = O.search(beta, omega, odd, 0.0, k, True, O.MOVES_REF, O.COUNT_ALL, greedy=True)
         assert (ca.expanded, ca.solutions) == (cg.expanded, cg.solutions)
     assert nonexist > 10
+
+
+def test_full_size_fixtures_are_the_oracles(O):
+    """The committed full-size answers (tests/golden/make_golden.py) are the
+    oracle's: all 10,000 n = 10 distances recomputed in both move sets, with
+    the odd-cycle bound <= d <= d(10) = 6, the reference filter never beating
+    the full move set and the distance histogram of the committed set; the
+    n = 16 depth-8 found / not-found on every 25th instance, the four
+    non-existence proofs at the positions the proofs fixture names, and the
+    exhaustive counts' tree sizes under the recorded cap."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                           "expected.json")) as f:
+        exp = json.load(f)
+    c1 = O.gen_uniform(0, 10, 10000)
+    ten = np.full(10000, 10, dtype=np.uint8)
+    nine = np.full(10000, 9, dtype=np.uint8)
+    dist = {}
+    for key, ms in (("all", O.MOVES_ALL), ("ref", O.MOVES_REF)):
+        res = O.search_batch_ex(ten, c1, nine, ms, O.FIRST, threads=8, iddfs=True)
+        d = np.array([r.depth for r in res])
+        assert "".join(map(str, d)) == exp["config1_distances"][key]
+        dist[key] = d
+    assert (dist["all"] <= dist["ref"]).all()
+    for q in range(0, 10000, 97):
+        beta, omega = O.linear_to_pair(c1[q], 10)
+        lb = (11 - O.count_odd_cycles(omega)) // 2
+        assert lb <= dist["all"][q] <= 6
+    assert list(np.bincount(dist["all"], minlength=7)) == [0, 0, 30, 477, 3268, 5536, 689]
+    # n = 16, move bound 8
+    _, _, p3 = O.gen_products(2, 16, 16, 9, 9, 1999)
+    perms = np.concatenate([p3, np.array([O.pack(list(range(15, -1, -1)))], dtype=np.uint64)])
+    idx = np.arange(0, 2000, 25)
+    res = O.search_batch_ex(np.full(len(idx), 16, np.uint8), perms[idx],
+                            np.full(len(idx), 8, np.uint8), O.MOVES_REF, O.FIRST, threads=8,
+                            greedy=True)
+    assert [r.found for r in res] == [int(exp["config3_found"][q]) for q in idx]
+    assert [q for q, c in enumerate(exp["config3_found"]) if c == "0"] == \
+        [p["q"] for p in exp["config3_nonexistence"]]
+    ca = exp["config3_count_all"]
+    assert all(r["expanded"] <= ca["cap"] for r in ca["rows"]) and len(ca["rows"]) <= ca["zero_slack"]
+    for r in ca["rows"]:
+        _, omega = O.linear_to_pair(r["perm"], 16)
+        assert (17 - O.count_odd_cycles(omega)) // 2 == 8  # zero slack at bound 8

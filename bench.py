@@ -458,6 +458,34 @@ def served_latency(native, rate, count, slots, queue_capacity, slot_bytes):
     }
 
 
+def served_saturation(native, count, slots, producers, window, plain_best):
+    """The served loop's capacity: `producers` host threads each keep up to
+    `window` requests of the served mix in flight (submit while fewer, else
+    wait for the oldest) with no pacing; then one open-loop burst offered
+    far above capacity.  Every witness verified."""
+    ns, ks, perms = native.gen_products(3, 12, 16, 6, 9, count)
+    reqs = build_requests(native, ns, ks, perms)
+    native.serve_start(device=0, slots=slots, queue_capacity=count, slot_bytes=1 << 30)
+    try:
+        native.serve_closed_loop(reqs[:4000], producers, window)  # warm
+        results, elapsed_us, st = native.serve_closed_loop(reqs, producers, window)
+        closed = {"jobs_per_s": round(count / (elapsed_us * 1e-6), 1),
+                  "p50_us": round(st.p50_us, 1), "p99_us": round(st.p99_us, 1),
+                  "mean_us": round(st.mean_us, 1)}
+        found, verified = verify(native, reqs, results)
+        burst = min(count, 100000)
+        out, el_us, st2 = native.serve_replay(reqs[:burst], [q * 0.25 for q in range(burst)])
+        open_rate = burst / (el_us * 1e-6)
+    finally:
+        native.serve_stop()
+    res = {"requests": count, "slots": slots, "producers": producers, "window": window,
+           "closed_loop": closed, "verified": verified, "found": found,
+           "open_loop_burst_jobs_per_s": round(open_rate, 1)}
+    if plain_best:
+        res["vs_plain_best"] = round(count / (elapsed_us * 1e-6) / plain_best, 2)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -569,6 +597,9 @@ def main() -> int:
         extra["served"] = served_latency(native, args.served_rate, args.served_requests,
                                          args.served_slots, 512, 1 << 30)
         log("served", extra["served"])
+        extra["served_saturation"] = served_saturation(native, 200000, args.served_slots, 8, 8,
+                                                       plain)
+        log("served saturation", extra["served_saturation"])
     if "cert" not in skip:
         extra["certificates"] = certificate_rates(native, reqs, run.results)
         log("certificates", extra["certificates"])

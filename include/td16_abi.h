@@ -337,12 +337,14 @@ int td16_serve_stats_get(td16_serve_stats *st, int reset_latency, char *msg, int
  * the host.  *elapsed_us receives the replay's wall time. */
 int td16_serve_replay(const td16_request *reqs, const double *arrival_us, int count,
                       td16_result *results, double *elapsed_us, char *msg, int msg_cap);
-/* Closed-loop load: `producers` host threads (1..1024) each submit a request
- * and wait for its result, then take the next (producer p takes reqs[p],
- * reqs[p + producers], ...), with no pacing -- the loop's capacity, as many
- * producer threads calling GPUSortingSearch.search would see it
- * (GPUSortingSearch.java:203-240).  Resets the latency window first. */
-int td16_serve_closed_loop(const td16_request *reqs, int count, int producers,
+/* Closed-loop load: `producers` host threads (1..1024), each keeping up to
+ * `window` (1..4096) of its requests in flight -- it submits while it has
+ * fewer, otherwise waits for its oldest -- with no pacing (producer p takes
+ * reqs[p], reqs[p + producers], ...).  window = 1 is a thread that calls
+ * search and blocks, as GPUSortingSearch's slot threads do
+ * (GPUSortingSearch.java:134-162, 203-240).  Measures the loop's capacity;
+ * resets the latency window first. */
+int td16_serve_closed_loop(const td16_request *reqs, int count, int producers, int window,
                            td16_result *results, double *elapsed_us, char *msg, int msg_cap);
 /* Stops the kernel after the admitted requests finish.  With a deadline, a
  * loop that does not drain and stop in time (or whose kernel failed) returns

@@ -183,7 +183,7 @@ SIGNATURES = {
     "td16_serve_search": (_i, [ctypes.POINTER(Request), ctypes.POINTER(Result), _c, _i]),
     "td16_serve_stats_get": (_i, [ctypes.POINTER(ServeStats), _i, _c, _i]),
     "td16_serve_stop": (_i, [_c, _i]),
-    "td16_serve_closed_loop": (_i, [ctypes.POINTER(Request), _i, _i, ctypes.POINTER(Result),
+    "td16_serve_closed_loop": (_i, [ctypes.POINTER(Request), _i, _i, _i, ctypes.POINTER(Result),
                                     ctypes.POINTER(ctypes.c_double), _c, _i]),
     "td16_serve_replay": (_i, [ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double), _i,
                                ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_double), _c, _i]),

@@ -357,6 +357,7 @@ void reset_batch_graph() {
 // condition variable, so a completion wakes exactly its waiter.
 struct Ticket {
   std::atomic<int> state{0};
+  std::atomic<bool> blocked{false};  // the waiter sleeps on cv (else it polls `state`)
   td16_result res;
   std::chrono::steady_clock::time_point t0;      // arrival
   std::chrono::steady_clock::time_point tAdmit;  // published to the device
@@ -507,9 +508,9 @@ void fail_all(Server *S, const char *why) {
     S->cvIdle.notify_all();
   }
   for (auto &t : lost) {
-    std::lock_guard<std::mutex> l(t->m);
     t->res.status = TD16_DEVICE_ERROR;
-    t->state.store(2, std::memory_order_release);
+    t->state.store(2, std::memory_order_seq_cst);
+    std::lock_guard<std::mutex> l(t->m);
     t->cv.notify_all();
   }
 }
@@ -588,12 +589,14 @@ void poll_loop(Server *S) {
       res.latency_us = us;
       res.queue_us = std::chrono::duration<double, std::micro>(t->tAdmit - t->t0).count();
       S->hist.add(us);
-      {
+      t->res = res;
+      // a waiter that polls sees the state; only one that sleeps needs a
+      // wake-up (state and flag are sequentially consistent on both sides)
+      t->state.store(1, std::memory_order_seq_cst);
+      if (t->blocked.load(std::memory_order_seq_cst)) {
         std::lock_guard<std::mutex> l(t->m);
-        t->res = res;
-        t->state.store(1, std::memory_order_release);
+        t->cv.notify_all();
       }
-      t->cv.notify_all();
     }
     S->nCompleted.fetch_add(done.size(), std::memory_order_relaxed);
   }
@@ -1264,11 +1267,13 @@ int td16_serve_start(const td16_serve_options *opt, char *msg, int cap) {
                           S->stream));
   Params P = make_params(S->buf, dev);
   P.served = 1;
-  // idle wavefronts sleep up to ~51 us: with a request arriving every few
-  // microseconds some wavefront is always awake to claim it, and the quiet
-  // memory system makes every step of a request faster (20k/s: p50 18 us,
-  // p99 42-51 us, against 92 / 141 us at ~3.2 us sleeps)
-  P.idleBackoff = 128;
+  // idle wavefronts sleep up to ~0.8 ms: among 1,024 of them one wakes
+  // about every microsecond, so a lone request is still picked up at once
+  // (20k/s: p50 18.5 us, p99 42 us, the same as with ~51 us sleeps), while
+  // the polling traffic that slows every request under load drops 16-fold
+  // (capacity at 16 / 64 slots: 284k / 114k requests/s with ~51 us sleeps,
+  // 736k / 597k with ~0.8 ms; against 92 / 141 us p50 / p99 at ~3.2 us)
+  P.idleBackoff = 2048;
   // served requests are mostly light and arrive while others run: checks
   // every 32 expansions keep the p99 lower (20k/s: 141-143 against 164-171
   // us at 16; the median is the same)
@@ -1374,24 +1379,38 @@ int serve_wait(const std::shared_ptr<Server> &S, uint64_t ticket, td16_result *r
     if (it == sh.map.end()) return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "unknown ticket");
     tk = it->second;
   }
-  // most results arrive within tens of microseconds: spin a little first
+  // Most results arrive within tens of microseconds.  The waiter polls its
+  // ticket -- spinning briefly, then yielding the core, so that with more
+  // waiting threads than cores the completion poller still runs -- and
+  // sleeps on the ticket only after about 2 ms.
   const auto start = std::chrono::steady_clock::now();
-  for (int q = 0; q < 2000 && tk->state.load(std::memory_order_acquire) == 0; ++q) {
+  auto late = [&] {
+    return S->deadlineMs > 0 &&
+           std::chrono::steady_clock::now() - start > std::chrono::milliseconds(S->deadlineMs);
+  };
+  for (int q = 0; q < 256 && tk->state.load(std::memory_order_acquire) == 0; ++q) {
 #if defined(__x86_64__)
     __builtin_ia32_pause();
 #endif
   }
-  {
+  while (tk->state.load(std::memory_order_acquire) == 0 &&
+         std::chrono::steady_clock::now() - start < std::chrono::milliseconds(2)) {
+    if (late())
+      // the ticket stays valid: a later wait may still collect the result
+      return set_msg(msg, cap, TD16_DEVICE_ERROR,
+                     "served loop: no result within the %d ms deadline", S->deadlineMs);
+    std::this_thread::yield();
+  }
+  if (tk->state.load(std::memory_order_acquire) == 0) {
     std::unique_lock<std::mutex> l(tk->m);
+    tk->blocked.store(true, std::memory_order_seq_cst);
     const int slice = S->deadlineMs > 0 ? std::min(S->deadlineMs, 20) : 20;
-    while (tk->state.load(std::memory_order_acquire) == 0) {
+    while (tk->state.load(std::memory_order_seq_cst) == 0) {
       tk->cv.wait_for(l, std::chrono::milliseconds(slice));
       if (tk->state.load(std::memory_order_acquire) != 0) break;
       if (S->failed.load())
         return set_msg(msg, cap, TD16_DEVICE_ERROR, "served loop: %s", S->failure);
-      if (S->deadlineMs > 0 &&
-          std::chrono::steady_clock::now() - start > std::chrono::milliseconds(S->deadlineMs))
-        // the ticket stays valid: a later wait may still collect the result
+      if (late())
         return set_msg(msg, cap, TD16_DEVICE_ERROR,
                        "served loop: no result within the %d ms deadline", S->deadlineMs);
     }
@@ -1484,7 +1503,7 @@ int td16_serve_replay(const td16_request *reqs, const double *arrivalUs, int cou
   return TD16_OK;
 }
 
-int td16_serve_closed_loop(const td16_request *reqs, int count, int producers,
+int td16_serve_closed_loop(const td16_request *reqs, int count, int producers, int window,
                            td16_result *results, double *elapsedUs, char *msg, int cap) {
   auto S = current_server();
   if (!S) return set_msg(msg, cap, TD16_NOT_INITIALIZED, "the served loop is not running");
@@ -1492,6 +1511,8 @@ int td16_serve_closed_loop(const td16_request *reqs, int count, int producers,
     return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "invalid closed-loop arguments");
   if (producers < 1 || producers > 1024)
     return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "producers must be in 1..1024");
+  if (window < 1 || window > 4096)
+    return set_msg(msg, cap, TD16_INVALID_ARGUMENT, "window must be in 1..4096");
   S->hist.reset();
   std::atomic<int> firstErr{TD16_OK};
   std::mutex errMu;
@@ -1501,20 +1522,31 @@ int td16_serve_closed_loop(const td16_request *reqs, int count, int producers,
   for (int p = 0; p < producers; ++p)
     ths.emplace_back([&, p] {
       char m[256];
-      for (int q = p; q < count; q += producers) {
-        if (firstErr.load(std::memory_order_relaxed) != TD16_OK) return;
-        uint64_t t = 0;
-        int st = serve_submit_at(S, &reqs[q], 0, &t, std::chrono::steady_clock::now(), m,
-                                 sizeof m);
-        if (st == TD16_OK) st = serve_wait(S, t, &results[q], m, sizeof m);
-        if (st != TD16_OK) {
-          int expected = TD16_OK;
-          if (firstErr.compare_exchange_strong(expected, st)) {
-            std::lock_guard<std::mutex> l(errMu);
-            err = m;
-          }
-          return;
+      // up to `window` requests of this producer in flight, collected in
+      // submission order
+      std::deque<std::pair<int, uint64_t>> open;
+      auto fail = [&](int st) {
+        int expected = TD16_OK;
+        if (firstErr.compare_exchange_strong(expected, st)) {
+          std::lock_guard<std::mutex> l(errMu);
+          err = m;
         }
+      };
+      for (int q = p; q < count || !open.empty();) {
+        if (firstErr.load(std::memory_order_relaxed) != TD16_OK) return;
+        if (q < count && (int)open.size() < window) {
+          uint64_t t = 0;
+          const int st = serve_submit_at(S, &reqs[q], 0, &t, std::chrono::steady_clock::now(),
+                                         m, sizeof m);
+          if (st != TD16_OK) return fail(st);
+          open.emplace_back(q, t);
+          q += producers;
+          continue;
+        }
+        const int st = serve_wait(S, open.front().second, &results[open.front().first], m,
+                                  sizeof m);
+        if (st != TD16_OK) return fail(st);
+        open.pop_front();
       }
     });
   for (auto &t : ths) t.join();

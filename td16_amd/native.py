@@ -261,15 +261,17 @@ def serve_replay(reqs: Sequence[Request], arrival_us: Sequence[float]):
     return out, el.value, serve_stats()
 
 
-def serve_closed_loop(reqs: Sequence[Request], producers: int):
-    """Closed-loop load: `producers` threads each submit and wait, with no
-    pacing.  Returns (results, elapsed_us, stats)."""
+def serve_closed_loop(reqs: Sequence[Request], producers: int, window: int = 1):
+    """Closed-loop load: `producers` threads each keep up to `window`
+    requests in flight, with no pacing.  Returns (results, elapsed_us,
+    stats)."""
     n = len(reqs)
     arr = (Request * max(1, n))(*reqs) if not isinstance(reqs, ctypes.Array) else reqs
     out = (Result * max(1, n))()
     el = ctypes.c_double(0)
     m = msgbuf()
-    check(lib().td16_serve_closed_loop(arr, n, producers, out, ctypes.byref(el), m, len(m)), m)
+    check(lib().td16_serve_closed_loop(arr, n, producers, window, out, ctypes.byref(el), m,
+                                       len(m)), m)
     return out, el.value, serve_stats()
 
 

@@ -180,6 +180,11 @@ typedef struct {
     int32_t host_threads;    /* host threads that prepare the requests and decode
                                 the results; 0 = default (all hardware threads,
                                 at most 16) */
+    int32_t count_candidates;/* 1 = run the counted build of the kernel, which
+                                also sums the candidate triples it evaluates
+                                (td16_batch_stats.candidates); for measurement:
+                                the default build does not count */
+    int32_t reserved;        /* 0 */
 } td16_batch_options;
 
 typedef struct {
@@ -198,6 +203,10 @@ typedef struct {
     double upload_ms;        /* device: the request table's copy to HBM */
     double decode_ms;        /* host: results of the last repeat to td16_result */
     double total_ms;         /* host wall time of the whole call */
+    uint64_t candidates;     /* count_candidates launches: candidate triples the last
+                                launch evaluated, C(m,3) per expanded node, m the
+                                positions it enumerates (breakpoints under the
+                                reference filter); 0 otherwise */
 } td16_batch_stats;
 
 /* Runs `count` requests in one persistent, work-sharing launch per repeat.

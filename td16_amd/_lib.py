@@ -99,6 +99,8 @@ class BatchOptions(ctypes.Structure):
         ("queue_bytes", ctypes.c_int64),
         ("check_every", ctypes.c_int32),
         ("host_threads", ctypes.c_int32),
+        ("count_candidates", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -115,6 +117,7 @@ class BatchStats(ctypes.Structure):
         ("upload_ms", ctypes.c_double),
         ("decode_ms", ctypes.c_double),
         ("total_ms", ctypes.c_double),
+        ("candidates", ctypes.c_uint64),
     ]
 
 

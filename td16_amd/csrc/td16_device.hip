@@ -186,6 +186,9 @@ struct BlockLds {
   uint16_t lex[kLexEntries];
   uint16_t lexPad[4];
   WaveLds wave[kWavesPerBlock];
+  // counted builds: candidate triples each wavefront evaluated (after the
+  // waves' state, so the layout the timed builds use does not move)
+  uint64_t cands[kWavesPerBlock];
 };
 
 // Scalar (wave-uniform) description of the item being searched.
@@ -260,7 +263,10 @@ __device__ void record_witness(const Params &P, Ctx &c, WaveLds &w, int depth, i
 
 // Expands the node `pi` at depth d into frame d.  Returns kExpandFound when
 // a witness was recorded (FIRST mode) and the item must stop.
-__device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, int d,
+// kCount: also add the node's candidate count to *cands (the counted builds).
+template <bool kCount>
+__device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, uint64_t *cands,
+                      int d,
                       uint64_t pi) {
   const int lane = lane_id();
   const int N = c.N, n = c.n;
@@ -336,6 +342,9 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   }
   m = uni(m);
   const int ncand = choose3(m);
+  if (kCount && lane == 0)
+    __hip_atomic_fetch_add(cands, (uint64_t)ncand, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WAVEFRONT);
   const int nchunks = (ncand + 63) >> 6;
   const int off = choose4(m);
   const int mode = c.flags & kFlagCountAll;
@@ -475,6 +484,12 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
     if (fd >= 0) break;
   }
   if (fd < 0) return 0;
+  if (P.served && P.itemCap > 0) {
+    // a served request keeps at most its slot's budget of items pending
+    const int pend = c.everShared ? lane0([&] { return ld_rlx(&P.states[c.job].pending); }) : 1;
+    if (P.itemCap - pend < want) want = P.itemCap - pend;
+    if (want < 1) return 0;
+  }
   Frame &F = w.frames[fd];
   const uint32_t fhdr = (uint32_t)uni((int)F.hdr);
   uint64_t bits = uni64(F.mask[ch]);
@@ -718,7 +733,8 @@ __device__ bool release_item(const Params &P, Ctx &c);
 
 // Runs one item (a node to expand and its whole subtree) to completion.
 // Returns true when this wave released the request's last pending item.
-__device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
+template <bool kCount>
+__device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, uint64_t *cands,
                          uint64_t pi, int depth0) {
   const int lane = lane_id();
   c.depth0 = depth0;
@@ -730,7 +746,7 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
 
   bool stop = false;
   if (c.shared && !countAll) stop = lane0([&] { return ld_rlx(&st->found); }) != 0;
-  if (!stop && expand(P, c, w, lex, depth0, pi) == kExpandFound) stop = true;
+  if (!stop && expand<kCount>(P, c, w, lex, cands, depth0, pi) == kExpandFound) stop = true;
   int d = depth0;
   while (!stop) {
     d = uni(d);
@@ -787,12 +803,6 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
         // heavy requests spread early instead of crawling on one wavefront
         // that shares its SIMD with light ones
         int64_t room = drained ? -bal : (c.expanded >= heavyAt ? kEarlyStock - bal : 0);
-        if (P.itemCap > 0) {
-          // a served request keeps at most its slot's budget of items pending
-          const int64_t pend = c.everShared ? ld_rlx(&P.states[c.job].pending) : 1;
-          const int64_t left = (int64_t)P.itemCap - pend;
-          if (left < room) room = left;
-        }
         if (room <= 0) return 0;
         return room < (int64_t)P.donateMax ? (int)room : P.donateMax;
       });
@@ -802,7 +812,7 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
       }
     }
     ++d;
-    if (expand(P, c, w, lex, d, child) == kExpandFound) break;
+    if (expand<kCount>(P, c, w, lex, cands, d, child) == kExpandFound) break;
   }
   __builtin_amdgcn_s_setprio(0);
   return release_item(P, c);
@@ -975,7 +985,7 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut, uint32_t &jwOut) 
     // Retrying at once turns a backlog into a storm of same-address CASes
     // (each ~12 ns at the memory side, every loser reloading and trying
     // again), which capped the loop near 100k requests/s with 128 or more
-    // requests admitted; with one attempt it serves 1.4-1.5M/s
+    // requests admitted; with one attempt it serves 0.7-1.5M/s
     if (r < avail && cas_rlx(&P.ctrl->consumed, r, r + 1)) got = 1;
   }
   got = uni(got);
@@ -1011,11 +1021,14 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut, uint32_t &jwOut) 
 
 // The persistent search loop of one block; the kernels below instantiate it
 // under two register budgets.
+template <bool kCount>
 __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
   for (int q = threadIdx.x; q < kLexEntries; q += blockDim.x) lds.lex[q] = P.lex[q];
   __syncthreads();
 
   const int wid = threadIdx.x >> 6;
+  uint64_t *cands = &lds.cands[wid];
+  if (kCount && (threadIdx.x & 63) == 0) *cands = 0;
   WaveLds &w = lds.wave[wid];
   const uint16_t *lex = lds.lex;
   const int lane = lane_id();
@@ -1097,7 +1110,10 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
                   ld_rlx(&P.ctrl->jobsDone) >= (uint32_t)cons) ? 1 : 0;
         });
       }
-      if (done) break;
+      if (done) {
+        if (kCount && lane == 0) add_rlx(&P.ctrl->candidates, *cands);
+        break;
+      }
       // idle polling touches a few words of HBM and contends with the
       // working wavefronts' atomics and cache write-backs; back off (about
       // 0.4 us per unit, up to P.idleBackoff units)
@@ -1162,7 +1178,7 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
     }
     load_job(P, c, w, j, limit);
     for (;;) {
-      if (!run_item(P, c, w, lex, pi, depth0)) break;
+      if (!run_item<kCount>(P, c, w, lex, cands, pi, depth0)) break;
       // last item of request j finished: next IDDFS limit or publish
       JobState *st = &P.states[j];
       int next = 0;
@@ -1207,7 +1223,7 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
 __global__ void __launch_bounds__(kBlockThreads, 4)
 td16_search_kernel(Params P) {
   __shared__ BlockLds lds;
-  search_blocks(P, lds);
+  search_blocks<false>(P, lds);
 }
 
 // Five waves per SIMD (96 VGPRs): the per-request paths spill about 40
@@ -1218,7 +1234,21 @@ td16_search_kernel(Params P) {
 __global__ void __launch_bounds__(kBlockThreads, 5)
 td16_search_kernel_w5(Params P) {
   __shared__ BlockLds lds;
-  search_blocks(P, lds);
+  search_blocks<false>(P, lds);
+}
+
+// Counted builds of both: the same search that also sums the candidate
+// triples it evaluates (Control::candidates); for measurement launches.
+__global__ void __launch_bounds__(kBlockThreads, 4)
+td16_search_kernel_counted(Params P) {
+  __shared__ BlockLds lds;
+  search_blocks<true>(P, lds);
+}
+
+__global__ void __launch_bounds__(kBlockThreads, 5)
+td16_search_kernel_w5_counted(Params P) {
+  __shared__ BlockLds lds;
+  search_blocks<true>(P, lds);
 }
 
 // Every shard's slot q starts out free for producer position q.
@@ -1228,7 +1258,13 @@ __global__ void td16_queue_init_kernel(uint64_t *seq, uint64_t capPerShard, uint
 }
 
 int launch_search(const Params &p, int blocks, bool wide, void *stream) {
-  if (wide)
+  if (p.countCandidates && wide)
+    hipLaunchKernelGGL(td16_search_kernel_w5_counted, dim3(blocks), dim3(kBlockThreads), 0,
+                       (hipStream_t)stream, p);
+  else if (p.countCandidates)
+    hipLaunchKernelGGL(td16_search_kernel_counted, dim3(blocks), dim3(kBlockThreads), 0,
+                       (hipStream_t)stream, p);
+  else if (wide)
     hipLaunchKernelGGL(td16_search_kernel_w5, dim3(blocks), dim3(kBlockThreads), 0,
                        (hipStream_t)stream, p);
   else

@@ -161,7 +161,8 @@ struct alignas(256) Control {
   uint64_t lastPoll;     // device clock of the last host-ring poll
   uint64_t submittedMirror;
   uint64_t stopMirror;   // device copy of ServeHost::submitted's stop bit
-  uint64_t pad5[27];
+  uint64_t candidates;   // candidate triples evaluated, summed at each wavefront's exit
+  uint64_t pad5[26];
   FeedShard feed[kFeedShards];
 };
 static_assert(offsetof(Control, single) == 128 && offsetof(Control, drained) == 256,
@@ -201,6 +202,7 @@ struct Params {
   const Job *hostJobs;   // served: device view of the host submit ring
   DevResult *hostResults;// served: device view of the host completion ring
   uint32_t *hostCompletionIdx; // served: ring index per completion sequence
+  int32_t countCandidates; // 1: run the counted build (sums Control::candidates)
 };
 
 // Launch wrappers (td16_device.hip).

@@ -180,6 +180,8 @@ extern "C" int td16_batch_noalloc(const td16_request *reqs, int count, td16_resu
   }
   std::vector<DevResult> out(nj);
   NA_HIP(hipMemcpy(out.data(), P.results, sizeof(DevResult) * nj, hipMemcpyDeviceToHost));
+  NA_HIP(hipMemcpy(&local.candidates, &P.ctrl->candidates, sizeof(uint64_t),
+                   hipMemcpyDeviceToHost));
   local.kernel_ms = wallMs / o.repeats;  // host-timed: no events are created
   local.wall_ms = wallMs / o.repeats;
   std::vector<uint32_t> pos;

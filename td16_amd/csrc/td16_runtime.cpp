@@ -184,6 +184,7 @@ void default_knobs(Params &P) {
 }
 
 void apply_batch_knobs(Params &P, const td16_batch_options &o) {
+  P.countCandidates = o.count_candidates ? 1 : 0;
   if (o.split_depth > 0) P.splitSlack = o.split_depth;
   if (o.donate_max > 0) P.donateMax = std::min(o.donate_max, 64);
   if (o.check_every > 0) P.checkEvery = std::min(o.check_every, 1024);
@@ -878,6 +879,8 @@ int launch_prepared(uint32_t nj, const uint32_t *pos, uint32_t count,
   }
   localStats.kernel_ms = kernelMs / o.repeats;
   localStats.wall_ms = wallMs / o.repeats;
+  TD16_HIP(hipMemcpy(&localStats.candidates, &b.dCtrl->candidates, sizeof(uint64_t),
+                     hipMemcpyDeviceToHost));
   const auto d0 = std::chrono::steady_clock::now();
   decode_batch(b.hResults, pos, count, results, o.host_threads, &localStats.expanded,
                &localStats.solutions, &localStats.donations);
@@ -1020,6 +1023,9 @@ int td16_batch_stream(const td16_request *reqs, int count, int chunk, td16_resul
     TD16_HIP(hipEventElapsedTime(&up, X.up0[s], X.up1[s]));
     total.kernel_ms += ms;
     total.upload_ms += up;
+    uint64_t cands = 0;
+    TD16_HIP(hipMemcpy(&cands, &X.buf[s].dCtrl->candidates, sizeof(cands), hipMemcpyDeviceToHost));
+    total.candidates += cands;
     const auto d0 = std::chrono::steady_clock::now();
     uint64_t e = 0, so = 0, dn = 0;
     decode_batch(X.buf[s].hResults, X.ws[s].pos.data(), (uint32_t)cnt[s], results + base[s],

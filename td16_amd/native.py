@@ -123,7 +123,7 @@ class BatchRun:
 def batch(reqs: Sequence[Request], device: int = 0, waves_per_cu: int = 0, split_depth: int = 0,
           donate_max: int = 0, use_graph: bool = False, repeats: int = 1,
           queue_bytes: int = 0, check_every: int = 0, host_threads: int = 0,
-          out: Optional[ctypes.Array] = None) -> BatchRun:
+          out: Optional[ctypes.Array] = None, count_candidates: bool = False) -> BatchRun:
     """One persistent launch over ``reqs`` (``td16_batch``).  ``out``: a
     result array of at least len(reqs) entries to fill (a new one
     otherwise)."""
@@ -132,7 +132,8 @@ def batch(reqs: Sequence[Request], device: int = 0, waves_per_cu: int = 0, split
     if out is None or len(out) < n:
         out = (Result * max(1, n))()
     opt = BatchOptions(device, waves_per_cu, split_depth, donate_max, 1 if use_graph else 0,
-                       repeats, queue_bytes, check_every, host_threads)
+                       repeats, queue_bytes, check_every, host_threads,
+                       1 if count_candidates else 0, 0)
     st = BatchStats()
     m = msgbuf()
     check(lib().td16_batch(arr, n, out, ctypes.byref(opt), ctypes.byref(st), m, len(m)), m)

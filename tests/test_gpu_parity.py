@@ -324,3 +324,20 @@ def test_prepare_thread_counts_give_the_same_batch(native):
     b = native.batch(reqs, host_threads=16)
     assert same_answers(native, reqs, a.results, b.results) > len(reqs) // 2
     assert a.stats.prepare_ms > 0 and b.stats.decode_ms > 0
+
+
+def test_counted_build_counts_and_keeps_answers(native):
+    """The counted build (count_candidates) finds what the timed build finds
+    and sums C(m,3) over the expanded nodes: exactly 1 for the reference's
+    smoke vector (one node, three breakpoints), and between one and C(N,3)
+    per expanded node on a mix."""
+    smoke = native.make_request([0, 1, 2], [1, 2, 0], 1, 2.0, 1)
+    run = native.batch([smoke], count_candidates=True)
+    assert run.results[0].found == 1 and run.stats.candidates == 1
+    reqs = mixed_batch(native, 47, 5000)
+    timed = native.batch(reqs)
+    counted = native.batch(reqs, count_candidates=True)
+    assert same_answers(native, reqs, timed.results, counted.results) > len(reqs) // 2
+    top = sum(counted.results[q].expanded * math.comb(reqs[q].n, 3) for q in range(len(reqs)))
+    assert counted.stats.expanded <= counted.stats.candidates <= top
+    assert timed.stats.candidates == 0

@@ -201,6 +201,50 @@ def cpu_baseline(native, ns, ks, perms, single_seconds=5.0):
     }
 
 
+def cpu_configs():
+    """The oracle on configurations 1 and 3, part of the CPU baseline: all
+    10,000 n = 10 exact distances (iterative deepening, all moves) and the
+    1,996 depth-8 searches of configuration 3 that find a witness, each on
+    every host core; and the 1,571,812-node non-existence proof of its
+    request 594 on one core (the oracle's proofs are single-threaded)."""
+    from oracle import oracle as O
+    cores = max(1, os.cpu_count() or 1)
+    c1 = O.gen_uniform(0, 10, 10000)
+    t0 = time.perf_counter()
+    res = O.search_batch_ex(np.full(10000, 10, dtype=np.uint8), c1,
+                            np.full(10000, 9, dtype=np.uint8), O.MOVES_ALL, O.FIRST,
+                            threads=cores, iddfs=True)
+    d1 = time.perf_counter() - t0
+    n1 = sum(r.expanded for r in res)
+    _, _, p3 = O.gen_products(2, 16, 16, 9, 9, 1999)
+    rev = sum(x << (4 * (15 - x)) for x in range(16))
+    p3 = np.concatenate([p3, np.array([rev], dtype=np.uint64)])
+    proofs = (220, 594, 1864, 1999)
+    keep = [q for q in range(2000) if q not in proofs]
+    s16 = np.full(len(keep), 16, dtype=np.uint8)
+    t0 = time.perf_counter()
+    res = O.search_batch_ex(s16, p3[keep], np.full(len(keep), 8, dtype=np.uint8), O.MOVES_REF,
+                            O.FIRST, threads=cores, greedy=True)
+    d3 = time.perf_counter() - t0
+    n3 = sum(r.expanded for r in res)
+    found3 = sum(1 for r in res if r.found == 1)
+    t0 = time.perf_counter()
+    res = O.search_batch_ex(s16[:1], p3[[594]], np.full(1, 8, dtype=np.uint8), O.MOVES_REF,
+                            O.FIRST, threads=1, greedy=True)
+    dp = time.perf_counter() - t0
+    return {
+        "cores": cores,
+        "config1_instances_per_s": round(10000 / d1, 1),
+        "config1_nodes_per_s": round(n1 / d1, 1),
+        "config3_found_instances": found3,
+        "config3_found_instances_per_s": round(len(keep) / d3, 1),
+        "config3_found_nodes_per_s": round(n3 / d3, 1),
+        "config3_proof594_nodes": int(res[0].expanded),
+        "config3_proof594_s_one_core": round(dp, 2),
+        "config3_proof594_nodes_per_s": round(res[0].expanded / dp, 1),
+    }
+
+
 def valu_peak(native, runs=3, iterations=20000):
     """Sustained int32 VALU rate of the device (median of `runs` probes)."""
     vals = sorted(native.probe_valu_peak(0, iterations) for _ in range(runs))
@@ -322,7 +366,7 @@ def config1(native, plain=True):
     return out
 
 
-def config3(native):
+def config3(native, plain=True):
     """Configuration 3: 2,000 n = 16 instances (1,999 products of 9 block
     transpositions, SplitMix64 seed 2, and the reverse permutation) at move
     bound 8 under the reference filter, in one batched launch; found /
@@ -354,6 +398,13 @@ def config3(native):
     exact = sum(1 for q, r in enumerate(rows)
                 if (crun.results[q].expanded, crun.results[q].solutions)
                 == (r["expanded"], r["solutions"]))
+    plain_legs = {}
+    if plain:
+        for slots in (1, 7):
+            rate, dt, res = plain_run(native, reqs, len(reqs), slots)
+            pf = np.array([r.found for r in res])
+            plain_legs["plain_%dslot_instances_per_s" % slots] = round(rate, 1)
+            plain_legs["plain_%dslot_found_equal_oracle" % slots] = int((pf == want).sum())
     return {
         "instances": 2000,
         "kernel_ms": round(st.kernel_ms, 4),
@@ -364,6 +415,7 @@ def config3(native):
         "found_equal_oracle": int((f["found"] == want).sum()),
         "nonexistence_proofs": len(proofs),
         "proofs_node_counts_exact": proofs_exact,
+        **plain_legs,
         "exhaustive_zero_slack": {
             "instances": len(rows),
             "of_zero_slack": exp["config3_count_all"]["zero_slack"],
@@ -638,12 +690,16 @@ def main() -> int:
         extra["config1"] = config1(native, plain="plain" not in skip)
         log("config1", extra["config1"])
     if "config3" not in skip:
-        extra["config3"] = config3(native)
+        extra["config3"] = config3(native, plain="plain" not in skip)
         log("config3", extra["config3"])
     if "served" not in skip:
         extra["served"] = served_latency(native, args.served_rate, args.served_requests,
                                          args.served_slots, 512, 1 << 30)
         log("served", extra["served"])
+        extra["served_sweep"] = [
+            served_latency(native, rate, int(rate * 0.4), slots, 512, 1 << 30)
+            for slots in (7, 64) for rate in (5000.0, 50000.0)]
+        log("served sweep", extra["served_sweep"])
         extra["served_saturation"] = served_saturation(native, 200000, args.served_slots, 8, 8,
                                                        plain)
         log("served saturation", extra["served_saturation"])
@@ -660,6 +716,7 @@ def main() -> int:
     cpu = None
     if "cpu" not in skip:
         cpu = cpu_baseline(native, ns, ks, perms)
+        cpu["configs"] = cpu_configs()
         log("cpu", cpu)
 
     traffic = None

@@ -203,7 +203,18 @@ struct Params {
   DevResult *hostResults;// served: device view of the host completion ring
   uint32_t *hostCompletionIdx; // served: ring index per completion sequence
   int32_t countCandidates; // 1: run the counted build (sums Control::candidates)
+  // batched feed: requests and requests per feed shard, the values init_feed
+  // writes into the control block, kept here too so that wavefronts compute
+  // shard bounds from kernel arguments instead of loading them
+  uint32_t numJobs;
+  uint32_t perShard;
 };
+
+// Copies the feed sizes init_feed wrote into `c` into the kernel arguments.
+inline void feed_params(Params &P, const Control &c) {
+  P.numJobs = c.numJobs;
+  P.perShard = c.perShard;
+}
 
 // Launch wrappers (td16_device.hip).
 // wide: the five-waves-per-SIMD build of the kernel (above 16 waves per CU)

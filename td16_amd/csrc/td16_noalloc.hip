@@ -168,6 +168,7 @@ extern "C" int td16_batch_noalloc(const td16_request *reqs, int count, td16_resu
   NA_HIP((hipError_t)launch_queue_init(P.seq, kRingPerShard, stream));
   std::memset(&gHostCtrl, 0, sizeof(gHostCtrl));
   const size_t ctrlBytes = init_feed(&gHostCtrl, nj);
+  feed_params(P, gHostCtrl);
 
   double wallMs = 0.0;
   for (int rep = 0; rep < o.repeats; ++rep) {

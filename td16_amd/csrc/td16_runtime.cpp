@@ -274,6 +274,7 @@ int run_plain(Slot &s, const td16_request *req, td16_result *res, char *msg, int
   TD16_HIP(hipMemcpyAsync(s.buf.dCtrl, s.buf.hCtrl, ctrlBytes, hipMemcpyHostToDevice,
                           s.stream));
   Params P = make_params(s.buf, s.dev);
+  feed_params(P, *s.buf.hCtrl);
   P.jobs = &s.buf.dCtrl->single;
   // one request: idle wavefronts only wait for its donations and its end, so
   // they poll every ~1.6 us instead of the batched ~13 us (measured: -5 us
@@ -837,6 +838,7 @@ int launch_prepared(uint32_t nj, const uint32_t *pos, uint32_t count,
   std::memset(b.hCtrl, 0, sizeof(Control));
   const size_t ctrlBytes = init_feed(b.hCtrl, nj);
   Params P = make_params(b, dev);
+  feed_params(P, *b.hCtrl);
   apply_batch_knobs(P, o);
 
   if (o.use_graph) {
@@ -1006,6 +1008,7 @@ int td16_batch_stream(const td16_request *reqs, int count, int chunk, td16_resul
     TD16_HIP(hipEventRecord(X.up1[s], X.copy));
     TD16_HIP(hipStreamWaitEvent(X.run, X.up1[s], 0));
     Params P = make_params(b, dev);
+    feed_params(P, *b.hCtrl);
     apply_batch_knobs(P, o);
     TD16_HIP(hipEventRecord(X.k0[s], X.run));
     TD16_HIP((hipError_t)launch_search(P, blocks, wide, X.run));

@@ -36,7 +36,24 @@ __global__ void td16_queue_init_kernel(uint64_t *seq, uint64_t capPerShard, uint
   if (i < total) seq[i] = i % capPerShard;
 }
 
-int launch_search(const Params &p, int blocks, bool wide, void *stream) {
+// Look-ahead claims (kLookAheadClaim requests held per atomic) save a claim
+// round trip per request, which pays when every wavefront runs many
+// requests; with few requests per wavefront a held request mostly leaves
+// another wavefront without work (all 10,000 n = 10 distances, 2.4 requests
+// per wavefront: 1.10 ms per launch without look-ahead against 1.20 ms with
+// it), so a batch of fewer than kLookAheadMinPerWave requests per wavefront
+// has none.  Otherwise the last 96 requests of each feed shard are claimed
+// one at a time: a request held there could wait behind a long search while
+// the feed drains.
+constexpr int kLookAheadMinPerWave = 8;
+int look_margin(const Params &p, int blocks) {
+  const uint64_t waves = (uint64_t)blocks * kWavesPerBlock;
+  return (uint64_t)p.numJobs >= kLookAheadMinPerWave * waves ? 96 : 0x3FFFFFFF;
+}
+
+int launch_search(const Params &p0, int blocks, bool wide, void *stream) {
+  Params p = p0;
+  p.lookMargin = look_margin(p0, blocks);
   if (p.countCandidates && wide)
     hipLaunchKernelGGL(td16_search_kernel_w5_counted, dim3(blocks), dim3(kBlockThreads), 0,
                        (hipStream_t)stream, p);

@@ -208,6 +208,9 @@ struct Params {
   // shard bounds from kernel arguments instead of loading them
   uint32_t numJobs;
   uint32_t perShard;
+  // batched feed: a request within lookMargin of its feed shard's end is
+  // not a look-ahead point (set by launch_search from the batch and grid)
+  int32_t lookMargin;
 };
 
 // Copies the feed sizes init_feed wrote into `c` into the kernel arguments.

@@ -231,7 +231,6 @@ struct Ctx {
   int preG;
   bool preValid;
   int preN;
-  int probeWait;   // batched feed: loop turns left before the next ring probe
   bool reorder;    // greedy order: frames visit the children gaining odd cycles first
   // request-level state (the request this wave started, or the item's)
   uint32_t jw;       // lane q < 32: word q of the request's 128-byte record
@@ -243,11 +242,13 @@ struct Ctx {
 
 enum { kExpandOk = 0, kExpandFound = 1 };
 
-// Nodes after which an item counts as heavy: it gets raised wave priority
-// and, while unstarted requests remain, keeps up to kEarlyStock of its
-// subtrees queued.  Requests without slack (kFlagTight) are where long
-// searches come from and qualify sooner; with a move bound of 8 or more
-// (tens of thousands of nodes on average) at the first check.
+// Nodes after which an item counts as heavy: while unstarted requests
+// remain, it keeps up to kEarlyStock of its subtrees queued.  Requests
+// without slack (kFlagTight) are where long searches come from and qualify
+// sooner; with a move bound of 8 or more (tens of thousands of nodes on
+// average) at the first check.  Any item past kPriorityNodes gets the SIMD's
+// issue priority.
+constexpr int kPriorityNodes = 64;
 constexpr int kHeavyNodes = 64;
 constexpr int kHeavyNodesDeep = 0;
 constexpr int kHeavyNodesSlack = 512;
@@ -368,13 +369,17 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   const int mode = c.flags & kFlagCountAll;
 
   // ---- candidates ------------------------------------------------------
+  // software-pipelined: the next chunk's triple-table entries are read while
+  // this chunk's cycle records are fetched
+  int eNext = lane < ncand ? (int)lex[off + lane] : 0;
   for (int ch = 0; ch < nchunks; ++ch) {
     const int t = (ch << 6) + lane;
     const bool valid = t < ncand;
-    const int e = valid ? (int)lex[off + t] : 0;
+    const int e = eNext;
     const int A = bperm(e & 31, einfo);
     const int B = bperm((e >> 5) & 31, einfo);
     const int C = bperm((e >> 10) & 31, einfo);
+    eNext = (t + 64) < ncand ? (int)lex[off + t + 64] : 0;
     // Branch-free odd-cycle delta of the triple.  Arcs between the chosen
     // positions along their cycles (meaningful for pairs on one cycle):
     const int Da = A & 31, Db = B & 31, Dc = C & 31;
@@ -638,12 +643,10 @@ __device__ __forceinline__ uint32_t feed_end(const Params &P, int s) {
   return e < nj ? (uint32_t)e : nj;
 }
 
-// Requests at the end of a feed shard are not claimed ahead.
-constexpr int kLookAheadMargin = 96;
 // Requests one look-ahead claim takes (one atomic on the shard's cursor).
+// The last Params::lookMargin requests of a feed shard are not claimed ahead
+// (set at launch from the batch and grid sizes).
 constexpr int kLookAheadClaim = 2;
-// Batched feed: loop turns between ring probes after a probe found nothing.
-constexpr int kProbeGap = 4;
 
 // Batched feed: claims the next unstarted request, from the home feed shard
 // first, then from whichever other shard still has some (every lane inspects
@@ -809,13 +812,14 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
     // every wavefront busy for free.
     if (++c.sinceCheck >= P.checkEvery) {
       c.sinceCheck = 0;
-      // an item that has proved heavy gets the SIMD's issue priority over
-      // the light requests sharing it, so it does not become the launch's
-      // tail; light work fills the remaining issue slots
+      // an item that has run long gets the SIMD's issue priority over the
+      // light requests sharing it, so it does not become the launch's tail
+      // (a request with slack can run a few hundred nodes without ever
+      // counting as heavy); light work fills the remaining issue slots
       const uint64_t heavyAt =
           !(c.flags & kFlagTight) ? kHeavyNodesSlack
           : job_max_depth(c) >= kDeepBound ? kHeavyNodesDeep : kHeavyNodes;
-      if (c.expanded >= heavyAt) __builtin_amdgcn_s_setprio(2);
+      if (c.expanded >= kPriorityNodes) __builtin_amdgcn_s_setprio(2);
       // balance = queued items - idle wavefronts (lanes 0..7, one shard
       // each) and whether requests are still waiting (lanes 8 and 9), in
       // one round trip
@@ -1071,7 +1075,6 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
   c.published = 0;
   c.preValid = false;
   c.preN = 0;
-  c.probeWait = 0;
   bool idle = false;
   int backoff = 1;
   for (;;) {
@@ -1079,20 +1082,12 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
     uint32_t j = 0;
     uint32_t servedJw = 0;
     // While the batched feed lasts, queued items are the early stock of
-    // heavy requests, taken before new requests.  Every device-scope access
-    // costs every wavefront here (the memory side serves them in turn, and
-    // its round trip grows with the number in flight: 1.9 us at 8 waves per
-    // CU, 3.8 us at 16), so a wavefront probes the rings only every
-    // kProbeGap-th turn after a probe that found nothing, and again at once
-    // after one that found an item.
-    bool tryPop = true;
+    // heavy requests, taken before new requests: a wavefront probes one ring
+    // shard at every turn (probing only every 4th turn after a miss saved
+    // nothing measurable on the batched mix and made the emitter-shaped
+    // stream 5 % slower, its heavy requests' stock waiting longer).
     const bool feeding = !P.served && !c.drained;
-    if (feeding) {
-      tryPop = c.probeWait == 0;
-      if (!tryPop) --c.probeWait;
-    }
-    bool haveItem = tryPop && pop_item(P, c, it, w, feeding) != 0;
-    if (tryPop && feeding) c.probeWait = haveItem ? 0 : kProbeGap - 1;
+    bool haveItem = pop_item(P, c, it, w, feeding) != 0;
     bool haveJob = false;
     if (!haveItem) {
       if (!P.served) {
@@ -1168,19 +1163,21 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
     uint64_t pi;
     int depth0, limit;
     if (haveJob) {
-      // look ahead while this request's record loads: once the last claim
-      // is used up, claim the home shard's next kLookAheadClaim requests
-      // with one atomic (unless this one is among the shard's last, where a
-      // held request could wait behind a long search)
+      c.jw = P.served ? servedJw : job_word(P, j);
+      limit = start_job(c);
+      // look ahead: once the last claim is used up, claim the home shard's
+      // next kLookAheadClaim requests with one atomic, whose result is read
+      // at the next request's start.  Not at a request without slack (a
+      // held request would wait behind what may be a long search) and not
+      // within lookMargin of the shard's end
       if (!P.served && !c.drained && !c.preValid && !c.homeDone &&
-          (int)j + kLookAheadMargin < (int)feed_end(P, c.home) &&
+          !((job_w7(c) >> 16) & kFlagTight) &&
+          (int64_t)j + P.lookMargin < (int64_t)feed_end(P, c.home) &&
           j >= (uint32_t)c.home * P.perShard) {
         if (lane == 0) c.preG = (int)add_rlx(&P.ctrl->feed[c.home].next, (uint32_t)kLookAheadClaim);
         c.preValid = true;
         c.preN = kLookAheadClaim;
       }
-      c.jw = P.served ? servedJw : job_word(P, j);
-      limit = start_job(c);
       pi = job_pi0(c);
       depth0 = 0;
       c.shared = false;

@@ -463,19 +463,29 @@ __device__ __forceinline__ int frame_move_lane(const Frame &F, uint32_t hdr, con
   return i | (j << 5) | (k << 10);
 }
 
-// Wave-uniform version (`hdr`, `ch` and `bit` are the same on every lane):
-// the lex entry and the position map are read at once, each lane holding
-// one position, and the three positions come back through readlane.
-__device__ __forceinline__ int frame_move(const Frame &F, uint32_t hdr, const uint16_t *lex,
-                                          int ch, int bit) {
-  const int lane = lane_id();
+// The frame as 8-byte words, as run_item reads it (one word per lane).
+constexpr int kFrameWords = (int)(sizeof(Frame) / 8);
+constexpr int kHdrWord = (int)(offsetof(Frame, hdr) / 8);
+static_assert(offsetof(Frame, hdr) % 8 == 0 && offsetof(Frame, pos) == offsetof(Frame, hdr) + 4 &&
+                  sizeof(Frame) % 8 == 0 && kFrameWords <= 64,
+              "run_item reads the frame by 8-byte word");
+
+// frame_move from a frame held one 8-byte word per lane (`fword`): position
+// of element e is byte e of the position map, which starts at the header
+// word's upper half.
+__device__ __forceinline__ int frame_pos(uint64_t fword, int e) {
+  const int b = e + 4;  // byte index from the header word's start
+  const uint64_t wv = (uint64_t)lane_value64((int64_t)fword, kHdrWord + (b >> 3));
+  return (int)((wv >> (8 * (b & 7))) & 0xFFu);
+}
+__device__ __forceinline__ int frame_move_words(uint64_t fword, uint32_t hdr, const uint16_t *lex,
+                                                int ch, int bit) {
   const int e = uni(lex[hdr_off(hdr) + (hdr_cand_chunk(hdr, ch) << 6) + bit]);
   int i = e & 31, j = (e >> 5) & 31, k = (e >> 10) & 31;
   if (!hdr_full(hdr)) {
-    const int pv = F.pos[lane < 20 ? lane : 0];
-    i = __builtin_amdgcn_readlane(pv, i);
-    j = __builtin_amdgcn_readlane(pv, j);
-    k = __builtin_amdgcn_readlane(pv, k);
+    i = frame_pos(fword, i);
+    j = frame_pos(fword, j);
+    k = frame_pos(fword, k);
   }
   return i | (j << 5) | (k << 10);
 }
@@ -777,27 +787,31 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
   if (c.shared && !countAll) stop = lane0([&] { return ld_rlx(&st->found); }) != 0;
   if (!stop && expand<kCount>(P, c, w, lex, cands, depth0, pi) == kExpandFound) stop = true;
   int d = depth0;
+  // the whole top frame in one LDS round trip: lane q < kFrameWords holds
+  // 8-byte word q (word 0 the sequence, words 1..22 the child masks, word 23
+  // the header and positions 0..3, words 24 and 25 positions 4..19)
+  const int fq = lane < kFrameWords ? lane : 0;
   while (!stop) {
     d = uni(d);
     Frame &F = w.frames[d];
-    // next expandable child of the top frame
-    const uint32_t hdr = (uint32_t)uni((int)F.hdr);
-    int ch = hdr_cursor(hdr);
-    const int nch = hdr_mask_chunks(hdr);
-    uint64_t bits = 0;
-    while (ch < nch) {
-      bits = uni64(F.mask[ch]);
-      if (bits) break;
-      ++ch;
-    }
-    if (ch >= nch) {
+    const uint64_t fword = reinterpret_cast<const uint64_t *>(&F)[fq];
+    const uint32_t hdr = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)fword, kHdrWord);
+    // next expandable child: the first nonzero mask chunk at or after the
+    // cursor (chunk q sits in lane q + 1)
+    const int q = lane - 1;
+    const uint64_t nz = ballot(q >= hdr_cursor(hdr) && q < hdr_mask_chunks(hdr) && fword != 0);
+    if (nz == 0) {
       if (d == depth0) break;
       --d;
       continue;
     }
+    const int chl = __builtin_ctzll(nz);
+    const int ch = chl - 1;
+    const uint64_t bits = (uint64_t)lane_value64((int64_t)fword, chl);
     const int bit = __builtin_ctzll(bits);
-    const int mv = frame_move(F, hdr, lex, ch, bit);
-    const uint64_t child = apply_move(uni64(F.pi), mv & 31, (mv >> 5) & 31, (mv >> 10) & 31);
+    const int mv = frame_move_words(fword, hdr, lex, ch, bit);
+    const uint64_t child = apply_move((uint64_t)lane_value64((int64_t)fword, 0), mv & 31,
+                                      (mv >> 5) & 31, (mv >> 10) & 31);
     if (lane == 0) {
       F.mask[ch] = bits & (bits - 1);
       F.hdr = (hdr & 0x00FFFFFFu) | ((uint32_t)ch << 24);

@@ -238,6 +238,7 @@ struct Ctx {
   uint64_t t0;       // device clock when the request started
   uint64_t totExpanded, totSolutions;  // finished items, while never shared
   int found, wdepth; // witness kept in LDS (w.wit), while never shared
+  int wmove, wlen;   // the witness an expansion just met: last move and length
 };
 
 enum { kExpandOk = 0, kExpandFound = 1 };
@@ -426,7 +427,11 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
           pk = F.pos[wk];
         }
         c.solutions += 1;
-        record_witness(P, c, w, cd, pi_ | (pj << 5) | (pk << 10));
+        // recorded by run_item once its DFS loop has ended: lane-dependent
+        // code on a loop's exit path makes the compiler treat the loop (and
+        // every value it carries) as divergent
+        c.wmove = uni(pi_ | (pj << 5) | (pk << 10));
+        c.wlen = cd;
         return kExpandFound;
       }
     }
@@ -696,11 +701,7 @@ __device__ int claim_job(const Params &P, Ctx &c) {
     // shards without requests are never read: the host does not reset them
     const bool has = (uint64_t)lane * per < nj && ld_rlx(&ctl->feed[lane].next) < shard_end((uint32_t)lane);
     const uint64_t avail = ballot(has) & ~tried;
-    if (avail == 0) {
-      c.drained = true;
-      if (lane == 0 && ld_rlx(&ctl->drained) == 0u) st_rlx(&ctl->drained, 1u);
-      return -1;
-    }
+    if (avail == 0) break;
     // first available shard after the home shard, cyclically
     const uint64_t rot = r ? ((avail >> r) | (avail << (64 - r))) : avail;
     const int pick = (__builtin_ctzll(rot) + r) & (kFeedShards - 1);
@@ -708,6 +709,9 @@ __device__ int claim_job(const Params &P, Ctx &c) {
     if (got >= 0) return got;
     tried |= 1ULL << pick;
   }
+  c.drained = true;
+  if (lane == 0 && ld_rlx(&ctl->drained) == 0u) st_rlx(&ctl->drained, 1u);
+  return -1;
 }
 
 // Lane q < 32 reads word q of request j's 128-byte record: one coalesced
@@ -776,6 +780,7 @@ template <bool kCount>
 __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, uint64_t *cands,
                          uint64_t pi, int depth0) {
   const int lane = lane_id();
+  depth0 = uni(depth0);
   c.depth0 = depth0;
   c.expanded = 0;
   c.solutions = 0;
@@ -784,8 +789,12 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
   JobState *st = &P.states[c.job];
 
   bool stop = false;
+  bool met = false;  // this item met a witness (FIRST)
   if (c.shared && !countAll) stop = lane0([&] { return ld_rlx(&st->found); }) != 0;
-  if (!stop && expand<kCount>(P, c, w, lex, cands, depth0, pi) == kExpandFound) stop = true;
+  if (!stop && uni(expand<kCount>(P, c, w, lex, cands, depth0, pi)) == kExpandFound) {
+    stop = true;
+    met = true;
+  }
   int d = depth0;
   // the whole top frame in one LDS round trip: lane q < kFrameWords holds
   // 8-byte word q (word 0 the sequence, words 1..22 the child masks, word 23
@@ -859,10 +868,14 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
       }
     }
     ++d;
-    if (expand<kCount>(P, c, w, lex, cands, d, child) == kExpandFound) break;
+    if (uni(expand<kCount>(P, c, w, lex, cands, d, child)) == kExpandFound) {
+      met = true;
+      break;
+    }
   }
+  if (met) record_witness(P, c, w, c.wlen, c.wmove);
   __builtin_amdgcn_s_setprio(0);
-  return release_item(P, c);
+  return uni(release_item(P, c) ? 1 : 0) != 0;
 }
 
 // Flushes the item's counters and releases it (the release publishes the
@@ -1073,7 +1086,9 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
   for (int q = threadIdx.x; q < kLexEntries; q += blockDim.x) lds.lex[q] = P.lex[q];
   __syncthreads();
 
-  const int wid = threadIdx.x >> 6;
+  // the wave's index in the block: the same on every lane, which the
+  // compiler cannot see in threadIdx.x, so it is made wave-uniform here
+  const int wid = uni((int)(threadIdx.x >> 6));
   uint64_t *cands = &lds.cands[wid];
   if (kCount && (threadIdx.x & 63) == 0) *cands = 0;
   WaveLds &w = lds.wave[wid];
@@ -1101,7 +1116,7 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
     // nothing measurable on the batched mix and made the emitter-shaped
     // stream 5 % slower, its heavy requests' stock waiting longer).
     const bool feeding = !P.served && !c.drained;
-    bool haveItem = pop_item(P, c, it, w, feeding) != 0;
+    bool haveItem = uni(pop_item(P, c, it, w, feeding)) != 0;
     bool haveJob = false;
     if (!haveItem) {
       if (!P.served) {
@@ -1117,13 +1132,13 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
             c.preValid = false;
           }
         }
-        if (jj < 0) jj = claim_job(P, c);
+        if (jj < 0) jj = uni(claim_job(P, c));
         if (jj >= 0) {
           j = (uint32_t)jj;
           haveJob = true;
         }
       } else {
-        haveJob = take_submitted(P, j, servedJw) != 0;
+        haveJob = uni(take_submitted(P, j, servedJw)) != 0;
       }
     }
     if (!haveItem && !haveJob) {
@@ -1214,6 +1229,7 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
       limit = uni(ld_rlx(&P.states[j].limit));  // ordered by pop_item's acquire
     }
     load_job(P, c, w, j, limit);
+    bool publish = false;
     for (;;) {
       if (!run_item<kCount>(P, c, w, lex, cands, pi, depth0)) break;
       // last item of request j finished: next IDDFS limit or publish
@@ -1239,8 +1255,9 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
         });
       }
       wave_sync();
+      next = uni(next);
       if (next == 0) {
-        publish_result(P, c, w, j);
+        publish = true;
         break;
       }
       load_job(P, c, w, j, next);
@@ -1248,6 +1265,7 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
       depth0 = 0;
       c.shared = false;
     }
+    if (publish) publish_result(P, c, w, j);
   }
 }
 

@@ -1107,6 +1107,21 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
   bool idle = false;
   int backoff = 1;
   for (;;) {
+    // The wave's loop-carried state is the same on every lane.  Said so
+    // here, the compiler keeps it in scalar registers and compiles every
+    // loop of the search as a uniform one: without this it cannot rule out
+    // that a lane-dependent branch (a lane-0 store, a per-lane probe) on some
+    // exit path left lanes disagreeing, and falls back to exec-mask loops,
+    // divergent phis and 114 VGPRs instead of 90.
+    c.preValid = uni(c.preValid ? 1 : 0) != 0;
+    c.preN = uni(c.preN);
+    c.drained = uni(c.drained ? 1 : 0) != 0;
+    c.homeDone = uni(c.homeDone ? 1 : 0) != 0;
+    idle = uni(idle ? 1 : 0) != 0;
+    backoff = uni(backoff);
+    c.shared = uni(c.shared ? 1 : 0) != 0;
+    c.everShared = uni(c.everShared ? 1 : 0) != 0;
+    c.published = (uint32_t)uni((int)c.published);
     Item it;
     uint32_t j = 0;
     uint32_t servedJw = 0;

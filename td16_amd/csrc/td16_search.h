@@ -37,7 +37,9 @@ namespace {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// (the builtin takes the predicate as a lane mask; HIP's __ballot widens it
+// to an int first, which costs a select and a compare per call)
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 __device__ __forceinline__ int bperm(int srcLane, int v) {
   return __builtin_amdgcn_ds_bpermute(srcLane << 2, v);
@@ -201,7 +203,10 @@ struct WaveLds {
 
 struct BlockLds {
   uint16_t lex[kLexEntries];
-  uint16_t lexPad[4];
+  // the candidate loop reads one chunk ahead, up to entry C(17,4) +
+  // 12 * 64 - 1 = 3147 for m = 17 (lanes past the candidate count discard
+  // what they read)
+  uint16_t lexPad[96];
   WaveLds wave[kWavesPerBlock];
   // counted builds: candidate triples each wavefront evaluated (after the
   // waves' state, so the layout the timed builds use does not move)
@@ -372,7 +377,7 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   // ---- candidates ------------------------------------------------------
   // software-pipelined: the next chunk's triple-table entries are read while
   // this chunk's cycle records are fetched
-  int eNext = lane < ncand ? (int)lex[off + lane] : 0;
+  int eNext = (int)lex[off + lane];
   for (int ch = 0; ch < nchunks; ++ch) {
     const int t = (ch << 6) + lane;
     const bool valid = t < ncand;
@@ -380,7 +385,7 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
     const int A = bperm(e & 31, einfo);
     const int B = bperm((e >> 5) & 31, einfo);
     const int C = bperm((e >> 10) & 31, einfo);
-    eNext = (t + 64) < ncand ? (int)lex[off + t + 64] : 0;
+    eNext = (int)lex[off + t + 64];
     // Branch-free odd-cycle delta of the triple.  Arcs between the chosen
     // positions along their cycles (meaningful for pairs on one cycle):
     const int Da = A & 31, Db = B & 31, Dc = C & 31;
@@ -1084,6 +1089,7 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut, uint32_t &jwOut) 
 template <bool kCount>
 __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
   for (int q = threadIdx.x; q < kLexEntries; q += blockDim.x) lds.lex[q] = P.lex[q];
+  if (threadIdx.x < 96) lds.lexPad[threadIdx.x] = 0;
   __syncthreads();
 
   // the wave's index in the block: the same on every lane, which the

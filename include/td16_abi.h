@@ -167,8 +167,8 @@ int td16_search_ex(int slot, const td16_request *req, td16_result *res,
 
 typedef struct {
     int32_t device;          /* HIP device ordinal */
-    int32_t waves_per_cu;    /* resident wavefronts per CU, in steps of 4; 0 = 16
-                                (tuned); 20 runs a 96-VGPR build of the kernel */
+    int32_t waves_per_cu;    /* resident wavefronts per CU, in steps of 4 up to 24;
+                                0 = 16; 24 runs an 80-VGPR build of the kernel */
     int32_t split_depth;     /* frames deeper than max_moves - split_depth are
                                 never donated; 0 = default (5) */
     int32_t donate_max;      /* children donated per donation (at most 64); 0 = default (16) */
@@ -274,7 +274,7 @@ typedef struct {
  * donated: the stack depth at which work is shared), donate_max (the steal
  * granularity) and check_every (how often a search looks for idle
  * wavefronts to share with) for this batch by timing it.  Coordinate
- * descent from `base` over waves_per_cu {4, 8, 12, 16, 20}, split_depth
+ * descent from `base` over waves_per_cu {4, 8, 12, 16, 20, 24}, split_depth
  * {2..7}, donate_max {4, 8, 16, 32, 64} and check_every {8, 16, 32}; each
  * point is one untimed launch and
  * `repeats` timed ones (median), and a point must beat the current choice by

@@ -1,16 +1,15 @@
-// Kernels of the persistent search (the body is in td16_search.h) at four
-// waves per SIMD, the queue initialisation kernel, and the launch wrappers
-// the host runtime calls.  The five-waves-per-SIMD builds are compiled in
-// td16_device_w5.hip.
+// Kernels of the persistent search (the body is in td16_search.h) at up to
+// five waves per SIMD, the queue initialisation kernel, and the launch
+// wrappers the host runtime calls.  The six-waves-per-SIMD builds are
+// compiled in td16_device_w6.hip.
 #include "td16_search.h"
 
 namespace td16 {
 
-// Register budget for four waves per SIMD (128 VGPRs): the few registers the
-// cap pushes to scratch belong to per-request set-up, not the per-node path.
-// The batched mix (about 6 nodes per request) runs fastest with it at 16
-// waves per CU.
-__global__ void __launch_bounds__(kBlockThreads, 4)
+// Register budget for five waves per SIMD (96 VGPRs; the body needs 90 and
+// nothing spills), so the same build runs 4 to 20 waves per CU.  The batched
+// mix (about 6 nodes per request) runs fastest at 20.
+__global__ void __launch_bounds__(kBlockThreads, 5)
 td16_search_kernel(Params P) {
   __shared__ BlockLds lds;
   search_blocks<false>(P, lds);
@@ -19,15 +18,15 @@ td16_search_kernel(Params P) {
 
 // Counted build: the same search that also sums the candidate triples it
 // evaluates (Control::candidates); for measurement launches.
-__global__ void __launch_bounds__(kBlockThreads, 4)
+__global__ void __launch_bounds__(kBlockThreads, 5)
 td16_search_kernel_counted(Params P) {
   __shared__ BlockLds lds;
   search_blocks<true>(P, lds);
 }
 
-// The five-waves-per-SIMD builds (td16_device_w5.hip).
-__global__ void td16_search_kernel_w5(Params P);
-__global__ void td16_search_kernel_w5_counted(Params P);
+// The six-waves-per-SIMD builds (td16_device_w6.hip).
+__global__ void td16_search_kernel_w6(Params P);
+__global__ void td16_search_kernel_w6_counted(Params P);
 
 
 // Every shard's slot q starts out free for producer position q.
@@ -55,13 +54,13 @@ int launch_search(const Params &p0, int blocks, bool wide, void *stream) {
   Params p = p0;
   p.lookMargin = look_margin(p0, blocks);
   if (p.countCandidates && wide)
-    hipLaunchKernelGGL(td16_search_kernel_w5_counted, dim3(blocks), dim3(kBlockThreads), 0,
+    hipLaunchKernelGGL(td16_search_kernel_w6_counted, dim3(blocks), dim3(kBlockThreads), 0,
                        (hipStream_t)stream, p);
   else if (p.countCandidates)
     hipLaunchKernelGGL(td16_search_kernel_counted, dim3(blocks), dim3(kBlockThreads), 0,
                        (hipStream_t)stream, p);
   else if (wide)
-    hipLaunchKernelGGL(td16_search_kernel_w5, dim3(blocks), dim3(kBlockThreads), 0,
+    hipLaunchKernelGGL(td16_search_kernel_w6, dim3(blocks), dim3(kBlockThreads), 0,
                        (hipStream_t)stream, p);
   else
     hipLaunchKernelGGL(td16_search_kernel, dim3(blocks), dim3(kBlockThreads), 0,
@@ -82,7 +81,7 @@ int search_kernel_occupancy(int *blocksPerCU, int *blocksPerCUWide) {
       blocksPerCU, td16_search_kernel, kBlockThreads, 0);
   if (e != hipSuccess) return (int)e;
   return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocksPerCUWide,
-                                                           td16_search_kernel_w5,
+                                                           td16_search_kernel_w6,
                                                            kBlockThreads, 0);
 }
 

@@ -220,7 +220,7 @@ inline void feed_params(Params &P, const Control &c) {
 }
 
 // Launch wrappers (td16_device.hip).
-// wide: the five-waves-per-SIMD build of the kernel (above 16 waves per CU)
+// wide: the six-waves-per-SIMD build of the kernel (above 20 waves per CU)
 int launch_search(const Params &p, int blocks, bool wide, void *stream);
 int launch_queue_init(uint64_t *seq, uint64_t capPerShard, void *stream);
 int search_kernel_occupancy(int *blocksPerCU, int *blocksPerCUWide);
@@ -237,7 +237,7 @@ void unpack_moves(const uint16_t *packed, int depth, int32_t (*moves)[3]);
 void default_knobs(Params &P);
 void apply_batch_knobs(Params &P, const td16_batch_options &o);
 // Persistent grid for `wavesPerCU` (0 = default) on a device with `cus` CUs:
-// blocks, and whether the five-waves-per-SIMD build runs.
+// blocks, and whether the six-waves-per-SIMD build runs.
 void batch_grid(int wavesPerCU, int cus, int blocksPerCU, int blocksPerCUWide, int *blocks,
                 bool *wide);
 void decode_result(const DevResult &r, td16_result *out);

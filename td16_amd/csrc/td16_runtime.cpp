@@ -49,8 +49,8 @@ namespace {
 struct DeviceInfo {
   int device = -1;
   int cus = 0;
-  int blocksPerCU = 0;      // resident blocks of the 4-waves-per-SIMD kernel
-  int blocksPerCUWide = 0;  // of the 5-waves-per-SIMD build
+  int blocksPerCU = 0;      // resident blocks of the 5-waves-per-SIMD kernel
+  int blocksPerCUWide = 0;  // of the 6-waves-per-SIMD build
   uint16_t *dLex = nullptr;
 };
 
@@ -192,11 +192,10 @@ void apply_batch_knobs(Params &P, const td16_batch_options &o) {
 
 void batch_grid(int wavesPerCU, int cus, int blocksPerCU, int blocksPerCUWide, int *blocks,
                 bool *wide) {
-  // 0 = the measured best for the batched mix: 16 waves per CU (4 per SIMD,
-  // the most the 128-VGPR build allows)
+  // 0 = 16 waves per CU (4 per SIMD)
   const int wantWaves = wavesPerCU > 0 ? wavesPerCU : 16;
-  // above the 4-waves-per-SIMD kernel's residency (16 waves per CU) the
-  // 5-waves-per-SIMD build runs
+  // above the main build's residency (20 waves per CU: its 96-VGPR budget
+  // allows 5 per SIMD) the 6-waves-per-SIMD build runs
   const int wantBlocks = std::max(1, wantWaves / kWavesPerBlock);
   *wide = wantBlocks > blocksPerCU && blocksPerCUWide > blocksPerCU;
   const int perCU = std::min(wantBlocks, *wide ? blocksPerCUWide : blocksPerCU);
@@ -1150,7 +1149,7 @@ int td16_autotune(const td16_request *reqs, int count, td16_result *results,
     return TD16_OK;
   };
 
-  static const int kWaves[] = {4, 8, 12, 16, 20};
+  static const int kWaves[] = {4, 8, 12, 16, 20, 24};
   static const int kSplit[] = {2, 3, 4, 5, 6, 7};
   static const int kDonate[] = {4, 8, 16, 32, 64};
   static const int kCheck[] = {8, 16, 32};

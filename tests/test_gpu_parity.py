@@ -230,8 +230,9 @@ KNOB_POINTS = [dict(waves_per_cu=4, split_depth=2, donate_max=4, check_every=1),
                dict(waves_per_cu=8, split_depth=7, donate_max=64, check_every=32),
                dict(waves_per_cu=16, split_depth=3, donate_max=16, check_every=8),
                dict(waves_per_cu=12, split_depth=5, donate_max=32, check_every=1024),
-               # the 5-waves-per-SIMD build of the kernel
-               dict(waves_per_cu=20, split_depth=4, donate_max=8, check_every=16)]
+               dict(waves_per_cu=20, split_depth=4, donate_max=8, check_every=16),
+               # the 6-waves-per-SIMD build of the kernel
+               dict(waves_per_cu=24, split_depth=6, donate_max=16, check_every=16)]
 KNOBS = ("waves_per_cu", "split_depth", "donate_max", "check_every")
 
 

@@ -190,8 +190,9 @@ typedef struct {
 typedef struct {
     double kernel_ms;        /* mean device time of one persistent launch */
     double wall_ms;          /* mean host wall time of one repeat: the control-block
-                                copy, the launch and the wait for its end (the
-                                requests' preparation and upload are not in it) */
+                                reset (a host copy, or inside the graph), the launch
+                                and the wait for its end (the requests' preparation
+                                and upload are not in it) */
     uint64_t expanded;       /* sum over requests (last repeat) */
     uint64_t solutions;
     uint64_t donations;      /* work items pushed to the shared queue */

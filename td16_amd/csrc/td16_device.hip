@@ -68,6 +68,40 @@ int launch_search(const Params &p0, int blocks, bool wide, void *stream) {
   return (int)hipGetLastError();
 }
 
+// Puts a batched launch's control block back in its starting state (what
+// init_feed and a zeroed block give): the feed cursors at their shards'
+// first requests and every counter at zero.  Captured ahead of the search in
+// a batch's hipGraph, so that a replay needs no host-to-device copy.
+__global__ void td16_control_reset_kernel(Control *c, uint32_t numJobs, uint32_t perShard) {
+  const int t = threadIdx.x;
+  if (t < kFeedShards) {
+    const uint64_t lo = (uint64_t)t * perShard;
+    c->feed[t].next = lo < numJobs ? (uint32_t)lo : numJobs;
+  }
+  if (t < kBalanceShards) {
+    c->balance[t].v = 0;
+    c->published[t].v = 0;
+  }
+  if (t == 0) {
+    c->numJobs = numJobs;
+    c->perShard = perShard;
+    c->drained = 0;
+    c->jobsDone = 0;
+    c->consumed = 0;
+    c->completions = 0;
+    c->lastPoll = 0;
+    c->submittedMirror = 0;
+    c->stopMirror = 0;
+    c->candidates = 0;
+  }
+}
+
+int launch_control_reset(Control *c, uint32_t numJobs, uint32_t perShard, void *stream) {
+  hipLaunchKernelGGL(td16_control_reset_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, c,
+                     numJobs, perShard);
+  return (int)hipGetLastError();
+}
+
 int launch_queue_init(uint64_t *seq, uint64_t capPerShard, void *stream) {
   const uint64_t total = capPerShard * kQueueShards;
   const uint64_t blocks = (total + 255) / 256;

@@ -223,6 +223,8 @@ inline void feed_params(Params &P, const Control &c) {
 // wide: the six-waves-per-SIMD build of the kernel (above 20 waves per CU)
 int launch_search(const Params &p, int blocks, bool wide, void *stream);
 int launch_queue_init(uint64_t *seq, uint64_t capPerShard, void *stream);
+// a batched launch's control block back to its starting state, on the device
+int launch_control_reset(Control *c, uint32_t numJobs, uint32_t perShard, void *stream);
 int search_kernel_occupancy(int *blocksPerCU, int *blocksPerCUWide);
 
 // Host helpers (td16_host.cpp).

@@ -379,8 +379,10 @@ def config3(native, plain=True):
     rev = sum(x << (4 * (15 - x)) for x in range(16))
     perms = list(p3) + [rev]
     reqs = (native.Request * 2000)(*[native.linear_request(int(p), 16, 8) for p in perms])
-    native.batch(reqs, waves_per_cu=20)
-    run = native.batch(reqs, waves_per_cu=20, repeats=3)
+    # the library defaults (16 waves per CU): the three long proofs set the
+    # launch's end (31-34 ms per launch, against 35-37 ms at 20 or 24)
+    native.batch(reqs)
+    run = native.batch(reqs, repeats=3)
     st = run.stats
     f = native.result_fields(run.results)
     want = np.frombuffer(exp["config3_found"].encode(), dtype=np.uint8) - ord("0")

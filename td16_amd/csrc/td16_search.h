@@ -201,7 +201,47 @@ struct WaveLds {
   uint16_t pad[4];
 };
 
+// The odd-cycle delta of a block transposition on positions a < b < c from
+// 12 bits of their cycle records (D: the position's distance from its
+// cycle's leader along the cycle, L: the cycle's length):
+//   bits 0-5   L_a&1, D_a&1, L_b&1, D_b&1, L_c&1, D_c&1
+//   bits 6-8   a,b on one cycle; b,c on one cycle; a,c on one cycle
+//   bits 9-11  D_a < D_b, D_b < D_c, D_c < D_a (the arc between two
+//              positions of one cycle wraps past the leader)
+// Three positions on one cycle in cyclic order (one wrap) split it into
+// three arcs; in the other order the cycle stays one.  Two on one cycle: the
+// arc between them closes on itself and the rest joins the third position's
+// cycle.  Three cycles merge into one.  Parities of the arcs: the distance
+// difference's, flipped when the arc wraps a cycle of odd length.  Code:
+// low two bits for the reference filter (0: delta 0, 1: +2, 2: -2, 3:
+// rejected: a three-cycle merge or a negative delta), the next two for the
+// full move set (never rejected).
+constexpr int kDeltaCodes = 4096;
+__device__ __forceinline__ uint32_t delta_code(int idx) {
+  const int la = idx & 1, qa = (idx >> 1) & 1, lb = (idx >> 2) & 1, qb = (idx >> 3) & 1;
+  const int lc = (idx >> 4) & 1, qc = (idx >> 5) & 1;
+  const int sab = (idx >> 6) & 1, sbc = (idx >> 7) & 1, sac = (idx >> 8) & 1;
+  const int wab = (idx >> 9) & 1, wbc = (idx >> 10) & 1, wca = (idx >> 11) & 1;
+  const int pab = qa ^ qb ^ (wab & la), pbc = qb ^ qc ^ (wbc & lb), pca = qc ^ qa ^ (wca & lc);
+  int delta;
+  if (sab && sbc) {
+    delta = (wab ^ wbc ^ wca) ? pab + pbc + pca - la : 0;
+  } else if (sab || sbc || sac) {
+    const int pd = sab ? pab : (sbc ? pbc : pca);
+    const int l1 = sbc ? lb : la;
+    const int l2 = la ^ lb ^ lc;
+    delta = pd + (l1 ^ l2 ^ pd) - l1 - l2;
+  } else {
+    delta = (la ^ lb ^ lc) - la - lb - lc;
+  }
+  const uint32_t cls = delta > 0 ? 1u : (delta < 0 ? 2u : 0u);
+  const bool refOk = (sab | sbc | sac) && delta >= 0;
+  return (refOk ? cls : 3u) | (cls << 2);
+}
+
 struct BlockLds {
+  // odd-cycle delta codes of a triple, by its 12 record bits (delta_code)
+  uint8_t dtab[kDeltaCodes];
   uint16_t lex[kLexEntries];
   // the candidate loop reads one chunk ahead, up to entry C(17,4) +
   // 12 * 64 - 1 = 3147 for m = 17 (lanes past the candidate count discard
@@ -246,7 +286,7 @@ struct Ctx {
   int wmove, wlen;   // the witness an expansion just met: last move and length
 };
 
-enum { kExpandOk = 0, kExpandFound = 1 };
+enum { kExpandOk = 0, kExpandFound = 1, kExpandLeaf = 2 };
 
 // Nodes after which an item counts as heavy: while unstarted requests
 // remain, it keeps up to kEarlyStock of its subtrees queued.  Requests
@@ -287,12 +327,13 @@ __device__ void record_witness(const Params &P, Ctx &c, WaveLds &w, int depth, i
 }
 
 // Expands the node `pi` at depth d into frame d.  Returns kExpandFound when
-// a witness was recorded (FIRST mode) and the item must stop.
+// a witness was recorded (FIRST mode) and the item must stop, kExpandLeaf
+// when the node has no expandable child (frame d is then left unwritten).
+// Each candidate's odd-cycle delta is one lookup in dtab (delta_code).
 // kCount: also add the node's candidate count to *cands (the counted builds).
 template <bool kCount>
-__device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, uint64_t *cands,
-                      int d,
-                      uint64_t pi) {
+__device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
+                      const uint8_t *dtab, uint64_t *cands, int d, uint64_t pi) {
   const int lane = lane_id();
   const int N = c.N, n = c.n;
   Frame &F = w.frames[d];
@@ -339,7 +380,9 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   const bool isLeader = (p == leader);
   const int ec = __builtin_popcountll(ballot(inN && isLeader && (L & 1)));
   const uint64_t bpMask = ballot(inN && nx != p);
-  const int info = D | (L << 8) | (leader << 16);
+  // the triple's table bits come from three copies of the record whose
+  // parity bits sit at different places (a: bits 8-9, b: 10-11, c: 12-13)
+  const int info = (D << 24) | (leader << 16) | (((L & 1) | ((D & 1) << 1)) << 8);
 
   // ---- enumeration set --------------------------------------------------
   const int cd = d + 1;
@@ -365,6 +408,8 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
     einfo = fperm(isBp ? r : 63, info);
     if (isBp) F.pos[r] = (uint8_t)p;
   }
+  const int einfoB = einfo + ((einfo & 0x300) * 3);
+  const int einfoC = einfo + ((einfo & 0x300) * 15);
   m = uni(m);
   const int ncand = choose3(m);
   if (kCount && lane == 0)
@@ -373,50 +418,46 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   const int nchunks = (ncand + 63) >> 6;
   const int off = choose4(m);
   const int mode = c.flags & kFlagCountAll;
+  // code -> witness / expandable child / greedy child, as 4-bit tables
+  // (codes 0, 1, 2: delta 0, +2, -2; code 3: rejected)
+  const int codeShift = allMoves ? 2 : 0;
+  uint32_t satC = 0, expC = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int cec = (ec + (k == 1 ? 2 : (k == 2 ? -2 : 0))) & 31;
+    const uint32_t s = (satMask >> cec) & 1u;
+    satC |= s << k;
+    expC |= (~s & (expMask >> cec) & 1u) << k;
+  }
+  const uint32_t grdC = c.reorder ? (expC & 2u) : 0u;
 
   // ---- candidates ------------------------------------------------------
   // software-pipelined: the next chunk's triple-table entries are read while
   // this chunk's cycle records are fetched
   int eNext = (int)lex[off + lane];
+  uint64_t anyExp = 0;
   for (int ch = 0; ch < nchunks; ++ch) {
     const int t = (ch << 6) + lane;
-    const bool valid = t < ncand;
     const int e = eNext;
     const int A = bperm(e & 31, einfo);
-    const int B = bperm((e >> 5) & 31, einfo);
-    const int C = bperm((e >> 10) & 31, einfo);
+    const int B = bperm((e >> 5) & 31, einfoB);
+    const int C = bperm((e >> 10) & 31, einfoC);
     eNext = (int)lex[off + t + 64];
-    // Branch-free odd-cycle delta of the triple.  Arcs between the chosen
-    // positions along their cycles (meaningful for pairs on one cycle):
-    const int Da = A & 31, Db = B & 31, Dc = C & 31;
-    const int La = (A >> 8) & 31, Lb = (B >> 8) & 31, Lc = (C >> 8) & 31;
-    const int la = La & 1, lb = Lb & 1, lc = Lc & 1;
-    const bool sab = (A >> 16) == (B >> 16), sbc = (B >> 16) == (C >> 16),
-               sac = (A >> 16) == (C >> 16);
-    int dab = Da - Db; dab += dab < 0 ? La : 0;
-    int dbc = Db - Dc; dbc += dbc < 0 ? Lb : 0;
-    int dca = Dc - Da; dca += dca < 0 ? Lc : 0;
-    // one cycle: order a->b->c splits it into the arcs dab, dbc, dca,
-    // a->c->b keeps one cycle
-    const int d1 = (dab + dbc + dca == La) ? (dab & 1) + (dbc & 1) + (dca & 1) - la : 0;
-    // x -> y on a shared cycle (parity l1), the third position alone on a
-    // cycle of parity l2: the x..y arc closes on itself, the rest joins the
-    // lone cycle
-    const int pd = (sab ? dab : (sbc ? dbc : dca)) & 1;
-    const int l1 = sbc ? lb : la;
-    const int l2 = la ^ lb ^ lc;
-    const int d2 = pd + (l1 ^ l2 ^ pd) - l1 - l2;
-    // three cycles merge into one
-    const int d3 = (la ^ lb ^ lc) - la - lb - lc;
-    const bool any2 = sab || sbc || sac;
-    const int delta = (sab && sbc) ? d1 : (any2 ? d2 : d3);
-    // the reference filter skips three-cycle merges and negative deltas
-    const bool acc = allMoves || (any2 && delta >= 0);
-    const int cec = (ec + delta) & 31;
-    const bool isSat = valid && acc && ((satMask >> cec) & 1u);
-    const bool isExp = valid && acc && !((satMask >> cec) & 1u) && ((expMask >> cec) & 1u);
-    const uint64_t satb = ballot(isSat);
-    const uint64_t expb = ballot(isExp);
+    // the table index: parity bits, same-cycle bits, wrap bits
+    const uint32_t ua = (uint32_t)A, ub = (uint32_t)B, uc = (uint32_t)C;
+    int idx = (int)(((ua | ub | uc) >> 8) & 63u);
+    idx |= ((ua >> 16) & 0xFFu) == ((ub >> 16) & 0xFFu) ? 64 : 0;
+    idx |= ((ub >> 16) & 0xFFu) == ((uc >> 16) & 0xFFu) ? 128 : 0;
+    idx |= ((ua >> 16) & 0xFFu) == ((uc >> 16) & 0xFFu) ? 256 : 0;
+    idx |= (ua >> 24) < (ub >> 24) ? 512 : 0;
+    idx |= (ub >> 24) < (uc >> 24) ? 1024 : 0;
+    idx |= (uc >> 24) < (ua >> 24) ? 2048 : 0;
+    const uint32_t code = ((uint32_t)dtab[idx] >> codeShift) & 3u;
+    // lanes past the candidate count drop out of the ballots
+    const int rem = ncand - (ch << 6);
+    const uint64_t vm = rem >= 64 ? ~0ULL : ((1ULL << rem) - 1ULL);
+    const uint64_t satb = ballot(((satC >> code) & 1u) != 0) & vm;
+    const uint64_t expb = ballot(((expC >> code) & 1u) != 0) & vm;
     if (satb != 0) {
       if (mode != 0) {
         c.solutions += (uint64_t)__builtin_popcountll(satb);
@@ -440,8 +481,9 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
         return kExpandFound;
       }
     }
+    anyExp |= expb;
     // reordered frame: the greedy children (two more odd cycles) first
-    const uint64_t greedy = c.reorder ? ballot(isExp && delta > 0) : 0ULL;
+    const uint64_t greedy = c.reorder ? (ballot(((grdC >> code) & 1u) != 0) & vm) : 0ULL;
     if (lane == 0) {
       if (c.reorder) {
         F.mask[ch] = greedy;
@@ -451,6 +493,9 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
       }
     }
   }
+  // a node without expandable children leaves its frame unread: the search
+  // resumes at the parent's
+  if (anyExp == 0) return kExpandLeaf;
   if (lane == 0) {
     F.pi = pi;
     F.hdr = hdr_make(off, nchunks, full, c.reorder, 0);
@@ -782,8 +827,8 @@ __device__ bool release_item(const Params &P, Ctx &c);
 // Runs one item (a node to expand and its whole subtree) to completion.
 // Returns true when this wave released the request's last pending item.
 template <bool kCount>
-__device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, uint64_t *cands,
-                         uint64_t pi, int depth0) {
+__device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
+                         const uint8_t *dtab, uint64_t *cands, uint64_t pi, int depth0) {
   const int lane = lane_id();
   depth0 = uni(depth0);
   c.depth0 = depth0;
@@ -796,9 +841,10 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
   bool stop = false;
   bool met = false;  // this item met a witness (FIRST)
   if (c.shared && !countAll) stop = lane0([&] { return ld_rlx(&st->found); }) != 0;
-  if (!stop && uni(expand<kCount>(P, c, w, lex, cands, depth0, pi)) == kExpandFound) {
-    stop = true;
-    met = true;
+  if (!stop) {
+    const int r0 = uni(expand<kCount>(P, c, w, lex, dtab, cands, depth0, pi));
+    stop = r0 != kExpandOk;
+    met = r0 == kExpandFound;
   }
   int d = depth0;
   // the whole top frame in one LDS round trip: lane q < kFrameWords holds
@@ -872,11 +918,12 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
         if (lane0([&] { return ld_rlx(&st->found); }) != 0) break;
       }
     }
-    ++d;
-    if (uni(expand<kCount>(P, c, w, lex, cands, d, child)) == kExpandFound) {
+    const int r = uni(expand<kCount>(P, c, w, lex, dtab, cands, d + 1, child));
+    if (r == kExpandFound) {
       met = true;
       break;
     }
+    if (r == kExpandOk) ++d;
   }
   if (met) record_witness(P, c, w, c.wlen, c.wmove);
   __builtin_amdgcn_s_setprio(0);
@@ -1089,6 +1136,7 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut, uint32_t &jwOut) 
 template <bool kCount>
 __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
   for (int q = threadIdx.x; q < kLexEntries; q += blockDim.x) lds.lex[q] = P.lex[q];
+  for (int q = threadIdx.x; q < kDeltaCodes; q += blockDim.x) lds.dtab[q] = (uint8_t)delta_code(q);
   if (threadIdx.x < 96) lds.lexPad[threadIdx.x] = 0;
   __syncthreads();
 
@@ -1252,7 +1300,7 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
     load_job(P, c, w, j, limit);
     bool publish = false;
     for (;;) {
-      if (!run_item<kCount>(P, c, w, lex, cands, pi, depth0)) break;
+      if (!run_item<kCount>(P, c, w, lex, lds.dtab, cands, pi, depth0)) break;
       // last item of request j finished: next IDDFS limit or publish
       JobState *st = &P.states[j];
       int next = 0;

@@ -346,15 +346,11 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
   const int shPrev = (4 * (p - 1)) & 63;
   const int sPrev = (p == 0) ? n : (int)((pi >> shPrev) & 15u);
   const int om = bperm(sPrev & 31, c.Rreg);  // omega(s_p) = R(s_{p-1})
-  int nx;
-  if (om == n) {
-    nx = n;
-  } else {
-    const uint64_t x = pi ^ ((uint64_t)(om & 15) * 0x1111111111111111ULL);
-    const uint64_t z = ((x - 0x1111111111111111ULL) & ~x & 0x8888888888888888ULL) |
-                       0x8000000000000000ULL;
-    nx = __builtin_ctzll(z) >> 2;
-  }
+  // position of symbol om in pi (the nibble equal to om), or n for om = n
+  const uint64_t x = pi ^ ((uint64_t)(om & 15) * 0x1111111111111111ULL);
+  const uint64_t z = ((x - 0x1111111111111111ULL) & ~x & 0x8888888888888888ULL) |
+                     0x8000000000000000ULL;
+  int nx = om == n ? n : (__builtin_ctzll(z) >> 2);
   nx = inN ? nx : p;  // lanes past N: harmless fixed points
   (void)sh;
 
@@ -362,12 +358,15 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
   // window is nx^i(p) for 0 <= i <= 2^r: M holds its members, J = nx^(2^r)(p)
   // and D the first i at which the window's smallest member (the cycle's
   // leader once the window covers the cycle) occurs.  One bpermute per round
-  // carries J | M << 5 | D << 22; four rounds cover cycles of up to 17.
+  // carries J | M << 5 | D << 22; four rounds cover cycles of up to 17, and
+  // the rounds stop once no window is still growing (before round r a window
+  // of fewer than 2^r + 1 members has closed on its cycle).
   int J = nx & 31;
   uint32_t M = (1u << (p & 31)) | (1u << J);
   int D = J < p ? 1 : 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
+    if (r >= 1 && ballot(__builtin_popcount(M) > (1u << r)) == 0) break;
     const uint32_t got =
         (uint32_t)bperm(J, (int)((uint32_t)J | ((M & 0x1FFFFu) << 5) | ((uint32_t)D << 22)));
     const uint32_t Mj = (got >> 5) & 0x1FFFFu;

@@ -125,6 +125,18 @@ __device__ __forceinline__ bool cas_rlx(T *p, T expected, T desired) {
 // never in polling loops.
 __device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+// Waits until this wave's earlier memory operations have completed and keeps
+// the compiler from moving memory operations across the wait.  Between
+// agent-scope atomics (performed at the memory side) and agent-scope atomic
+// stores (written through to memory) this orders them for every other wave
+// without the L2 write-back of fence_rel; plain stores still need fence_rel.
+// (A workgroup-scope release fence emits no wait at all outside
+// thread-group-split mode, so the wait is spelled out.)
+__device__ __forceinline__ void complete_mem() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 
 // Lane 0 runs f, every lane gets its int result.
@@ -317,13 +329,15 @@ __device__ void record_witness(const Params &P, Ctx &c, WaveLds &w, int depth, i
   }
   int won = 1;
   if (c.shared) won = lane0([&] { return cas_rlx(&st->found, 0, 1) ? 1 : 0; });
-  else if (lane == 0) st->found = 1;
+  else if (lane == 0) st_rlx(&st->found, 1);
   if (!won) return;
-  if (lane < depth) {
-    const uint16_t mv = (lane == depth - 1) ? (uint16_t)lastMove : w.path[lane];
-    st->moves[lane] = mv;
-  }
-  if (lane == 0) st->depth = depth;
+  // agent-scope stores, two moves per word: the item's release of `pending`
+  // (release_item) then needs only to wait for them
+  const int q0 = 2 * lane, q1 = 2 * lane + 1;
+  const uint32_t m0 = q0 < depth ? (q0 == depth - 1 ? (uint32_t)lastMove : (uint32_t)w.path[q0]) : 0u;
+  const uint32_t m1 = q1 < depth ? (q1 == depth - 1 ? (uint32_t)lastMove : (uint32_t)w.path[q1]) : 0u;
+  if (q0 < depth) st_rlx(reinterpret_cast<uint32_t *>(st->moves) + lane, m0 | (m1 << 16));
+  if (lane == 0) st_rlx(&st->depth, depth);
 }
 
 // Expands the node `pi` at depth d into frame d.  Returns kExpandFound when
@@ -619,16 +633,16 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
     if (!c.everShared) {
       // first donation: the request's shared state is written only now,
       // from registers (a FIRST search still running has no witness yet;
-      // the counts of finished IDDFS rounds join here), and the fence below
-      // publishes it with the items
-      st->pending = 1 + cnt;
-      st->found = 0;
-      st->limit = c.limit;
-      st->split = cnt;
-      st->expanded = c.totExpanded;
-      st->solutions = c.totSolutions;
-      st->depth = 0;
-      st->t0 = c.t0;
+      // the counts of finished IDDFS rounds join here) with agent-scope
+      // stores, and the fence below publishes the items
+      st_rlx(&st->pending, 1 + cnt);
+      st_rlx(&st->found, 0);
+      st_rlx(&st->limit, c.limit);
+      st_rlx(&st->split, cnt);
+      st_rlx(&st->expanded, c.totExpanded);
+      st_rlx(&st->solutions, c.totSolutions);
+      st_rlx(&st->depth, 0);
+      st_rlx(&st->t0, c.t0);
     } else {
       add_rlx(&st->pending, cnt);
       add_rlx(&st->split, cnt);
@@ -674,7 +688,8 @@ __device__ int pop_item(const Params &P, Ctx &c, Item &it, WaveLds &w, bool feed
   bool ready = false;
   uint64_t hl = 0;
   if (lane < (feeding ? 1 : 2)) {
-    // relaxed polling: the acquire fence is paid only once a slot is claimed
+    // relaxed polling (served: the acquire fence is paid only once a slot is
+    // claimed)
     hl = ld_rlx(&P.queue[myShard].head);
     ready = ld_rlx(&P.seq[(uint64_t)myShard * cap + hl % cap]) == hl + 1;
   }
@@ -685,10 +700,23 @@ __device__ int pop_item(const Params &P, Ctx &c, Item &it, WaveLds &w, bool feed
   const uint64_t h = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(hl >> 32), src) << 32) |
                      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hl, src);
   if (lane0([&] { return cas_rlx(&P.queue[shard].head, h, h + 1) ? 1 : 0; }) == 0) return 0;
-  fence_acq();
+  // The record is read with agent-scope loads (lane q < 9: word q — pi, job,
+  // depth, moves), which the producer's release made current, so the
+  // batched path needs no acquire: an agent-scope acquire invalidates the
+  // XCD's L2, and with many hand-offs in flight those invalidations (and the
+  // producers' write-backs) stalled every wave's memory traffic (a probe
+  // took ~16 us on configuration 3).  Served: the receiver reads the
+  // request's job-table copy with plain loads, so it still acquires.
+  if (P.served) fence_acq();
   const uint64_t slot = (uint64_t)shard * cap + h % cap;
-  it = P.items[slot];
-  if (lane < 10) w.path[lane] = P.items[slot].moves[lane];
+  uint32_t *srcw = reinterpret_cast<uint32_t *>(&P.items[slot]);
+  const uint32_t word = lane < 9 ? ld_rlx(srcw + lane) : 0u;
+  it.pi = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)word, 1) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((int)word, 0);
+  it.job = (uint32_t)__builtin_amdgcn_readlane((int)word, 2);
+  it.depth = (uint32_t)__builtin_amdgcn_readlane((int)word, 3);
+  const uint32_t mw = (uint32_t)bperm(4 + ((lane & 15) >> 1), (int)word);
+  if (lane < 10) w.path[lane] = (uint16_t)(mw >> (16 * (lane & 1)));
   // the item is in registers once every load has returned: free the slot
   __builtin_amdgcn_s_waitcnt(0);
   if (lane == 0) st_rlx(&P.seq[slot], h + cap);
@@ -957,7 +985,11 @@ __device__ bool release_item(const Params &P, Ctx &c) {
   if (lane == 0) {
     if (expanded) add_rlx(&st->expanded, expanded);
     if (c.solutions) add_rlx(&st->solutions, c.solutions);
-    last = __hip_atomic_fetch_add(&st->pending, -1, __ATOMIC_RELEASE,
+    // every write an item makes to its request's state is an agent-scope
+    // atomic or atomic store (counts, witness, limit): waiting for them to
+    // complete orders them before the count below, no L2 write-back needed
+    complete_mem();
+    last = __hip_atomic_fetch_add(&st->pending, -1, __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_AGENT) == 1 ? 1 : 0;
   }
   last = uni(last);
@@ -1307,16 +1339,16 @@ __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
         if ((c.flags & kFlagIddfs) && !c.found && c.limit < job_max_depth(c)) {
           next = c.limit + 1;
           if (lane == 0) {
-            st->limit = next;
-            st->pending = 1;
+            st_rlx(&st->limit, next);
+            st_rlx(&st->pending, 1);
           }
         }
       } else {
         next = lane0([&] {
           if ((c.flags & kFlagIddfs) && st->found == 0 && st->limit < job_max_depth(c)) {
             const int nl = st->limit + 1;
-            st->limit = nl;
-            st->pending = 1;
+            st_rlx(&st->limit, nl);
+            st_rlx(&st->pending, 1);
             return nl;
           }
           return 0;

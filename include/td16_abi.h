@@ -168,7 +168,8 @@ int td16_search_ex(int slot, const td16_request *req, td16_result *res,
 typedef struct {
     int32_t device;          /* HIP device ordinal */
     int32_t waves_per_cu;    /* resident wavefronts per CU, in steps of 4 up to 24;
-                                0 = 16; 24 runs an 80-VGPR build of the kernel */
+                                0 = 20 for a batch of at least 128 requests per CU,
+                                else 16; 24 runs an 80-VGPR build of the kernel */
     int32_t split_depth;     /* frames deeper than max_moves - split_depth are
                                 never donated; 0 = default (5) */
     int32_t donate_max;      /* children donated per donation (at most 64); 0 = default (16) */

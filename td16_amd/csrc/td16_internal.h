@@ -238,10 +238,11 @@ void unpack_moves(const uint16_t *packed, int depth, int32_t (*moves)[3]);
 // The library's default sharing knobs, and a batch's overrides of them.
 void default_knobs(Params &P);
 void apply_batch_knobs(Params &P, const td16_batch_options &o);
-// Persistent grid for `wavesPerCU` (0 = default) on a device with `cus` CUs:
-// blocks, and whether the six-waves-per-SIMD build runs.
-void batch_grid(int wavesPerCU, int cus, int blocksPerCU, int blocksPerCUWide, int *blocks,
-                bool *wide);
+// Persistent grid for `wavesPerCU` (0 = default for a batch of `numJobs`
+// requests) on a device with `cus` CUs: blocks, and whether the
+// six-waves-per-SIMD build runs.
+void batch_grid(int wavesPerCU, uint32_t numJobs, int cus, int blocksPerCU, int blocksPerCUWide,
+                int *blocks, bool *wide);
 void decode_result(const DevResult &r, td16_result *out);
 void trivial_result(int trivial, td16_result *out);
 

@@ -133,7 +133,8 @@ extern "C" int td16_batch_noalloc(const td16_request *reqs, int count, td16_resu
   std::memset(&local, 0, sizeof(local));
   int blocks = 0;
   bool wide = false;
-  batch_grid(o.waves_per_cu, d.cus, d.blocksPerCU, d.blocksPerCUWide, &blocks, &wide);
+  batch_grid(o.waves_per_cu, (uint32_t)count, d.cus, d.blocksPerCU, d.blocksPerCUWide, &blocks,
+             &wide);
   local.blocks = blocks;
   local.waves = blocks * kWavesPerBlock;
   if (nj == 0) {

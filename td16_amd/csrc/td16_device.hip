@@ -47,7 +47,7 @@ __global__ void td16_queue_init_kernel(uint64_t *seq, uint64_t capPerShard, uint
 constexpr int kLookAheadMinPerWave = 8;
 int look_margin(const Params &p, int blocks) {
   const uint64_t waves = (uint64_t)blocks * kWavesPerBlock;
-  return (uint64_t)p.numJobs >= kLookAheadMinPerWave * waves ? 96 : 0x3FFFFFFF;
+  return (uint64_t)p.numJobs >= kLookAheadMinPerWave * waves ? 96 : kNoLookAhead;
 }
 
 int launch_search(const Params &p0, int blocks, bool wide, void *stream) {

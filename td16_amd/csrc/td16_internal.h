@@ -181,6 +181,10 @@ struct alignas(128) ServeHost {
 };
 constexpr uint64_t kServeStop = 1ULL << 63;
 
+// Params::lookMargin of a batch with too few requests per wavefront for
+// look-ahead claims (no request is ever within it of a shard's end).
+constexpr int32_t kNoLookAhead = 0x3FFFFFFF;
+
 struct Params {
   const Job *jobs;
   JobState *states;
@@ -209,7 +213,8 @@ struct Params {
   uint32_t numJobs;
   uint32_t perShard;
   // batched feed: a request within lookMargin of its feed shard's end is
-  // not a look-ahead point (set by launch_search from the batch and grid)
+  // not a look-ahead point (set by launch_search from the batch and grid;
+  // kNoLookAhead for a batch of few requests per wavefront)
   int32_t lookMargin;
 };
 

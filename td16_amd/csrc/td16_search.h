@@ -252,8 +252,10 @@ __device__ __forceinline__ uint32_t delta_code(int idx) {
 }
 
 struct BlockLds {
-  // odd-cycle delta codes of a triple, by its 12 record bits (delta_code)
-  uint8_t dtab[kDeltaCodes];
+  // odd-cycle delta codes of a triple, by its 12 record bits (delta_code);
+  // entry kDeltaCodes (and the padding) is code 3 in both move sets, read by
+  // lanes past a node's candidate count
+  uint8_t dtab[kDeltaCodes + 8];
   uint16_t lex[kLexEntries];
   // the candidate loop reads one chunk ahead, up to entry C(17,4) +
   // 12 * 64 - 1 = 3147 for m = 17 (lanes past the candidate count discard
@@ -443,6 +445,7 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
     expC |= (~s & (expMask >> cec) & 1u) << k;
   }
   const uint32_t grdC = c.reorder ? (expC & 2u) : 0u;
+  const uint64_t plainMask = c.reorder ? 0ULL : ~0ULL;
 
   // ---- candidates ------------------------------------------------------
   // software-pipelined: the next chunk's triple-table entries are read while
@@ -465,12 +468,12 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
     idx |= (ua >> 24) < (ub >> 24) ? 512 : 0;
     idx |= (ub >> 24) < (uc >> 24) ? 1024 : 0;
     idx |= (uc >> 24) < (ua >> 24) ? 2048 : 0;
+    // lanes past the candidate count read code 3 and drop out of the ballots
+    // (a vector select: masking the ballots cost scalar instructions)
+    if (lane >= ncand - (ch << 6)) idx = kDeltaCodes;
     const uint32_t code = ((uint32_t)dtab[idx] >> codeShift) & 3u;
-    // lanes past the candidate count drop out of the ballots
-    const int rem = ncand - (ch << 6);
-    const uint64_t vm = rem >= 64 ? ~0ULL : ((1ULL << rem) - 1ULL);
-    const uint64_t satb = ballot(((satC >> code) & 1u) != 0) & vm;
-    const uint64_t expb = ballot(((expC >> code) & 1u) != 0) & vm;
+    const uint64_t satb = ballot(((satC >> code) & 1u) != 0);
+    const uint64_t expb = ballot(((expC >> code) & 1u) != 0);
     if (satb != 0) {
       if (mode != 0) {
         c.solutions += (uint64_t)__builtin_popcountll(satb);
@@ -495,15 +498,13 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
       }
     }
     anyExp |= expb;
-    // reordered frame: the greedy children (two more odd cycles) first
-    const uint64_t greedy = c.reorder ? (ballot(((grdC >> code) & 1u) != 0) & vm) : 0ULL;
+    // reordered frame: the greedy children (two more odd cycles) first; a
+    // frame in candidate order keeps them all in the first nchunks masks
+    // (the second half is then never scanned).  No branch on the order.
+    const uint64_t greedy = ballot(((grdC >> code) & 1u) != 0);
     if (lane == 0) {
-      if (c.reorder) {
-        F.mask[ch] = greedy;
-        F.mask[nchunks + ch] = expb & ~greedy;
-      } else {
-        F.mask[ch] = expb;
-      }
+      F.mask[ch] = greedy | (expb & plainMask);
+      F.mask[nchunks + ch] = expb & ~greedy;
     }
   }
   // a node without expandable children leaves its frame unread: the search
@@ -1183,7 +1184,8 @@ __device__ int take_submitted(const Params &P, uint32_t &jOut, uint32_t &jwOut) 
 template <bool kCount>
 __device__ __forceinline__ void search_blocks(Params P, BlockLds &lds) {
   for (int q = threadIdx.x; q < kLexEntries; q += blockDim.x) lds.lex[q] = P.lex[q];
-  for (int q = threadIdx.x; q < kDeltaCodes; q += blockDim.x) lds.dtab[q] = (uint8_t)delta_code(q);
+  for (int q = threadIdx.x; q < kDeltaCodes + 8; q += blockDim.x)
+    lds.dtab[q] = q < kDeltaCodes ? (uint8_t)delta_code(q) : (uint8_t)0xFFu;
   if (threadIdx.x < 96) lds.lexPad[threadIdx.x] = 0;
   __syncthreads();
 

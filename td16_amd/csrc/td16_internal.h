@@ -24,6 +24,7 @@ namespace td16 {
 constexpr int kMaxN = 17;
 constexpr int kMaxDepth = 9;
 constexpr int kMaxChunks = 11;      // ceil(C(17,3) / 64)
+static_assert(kMaxChunks <= 32, "expand() keeps a node's masks in lanes ch and 32 + ch");
 constexpr int kLexEntries = 3060;   // sum over m = 3..17 of C(m,3) = C(18,4)
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlockThreads = 64 * kWavesPerBlock;

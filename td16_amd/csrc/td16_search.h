@@ -66,6 +66,13 @@ __device__ __forceinline__ uint64_t uni64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// v = value in lane `lane` of v, the other lanes unchanged (v_writelane_b32,
+// lane select in M0; this compiler has no builtin for it).  `value` and
+// `lane` are wave-uniform.
+__device__ __forceinline__ void write_lane(int &v, uint32_t value, int lane) {
+  asm volatile("v_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(value), "{m0}"(lane));
+}
+
 __device__ __forceinline__ int rank_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
@@ -176,7 +183,7 @@ __device__ __forceinline__ int choose4(int m) {
 // others, so a visit in chunk order tries the greedy moves first.
 struct Frame {
   uint64_t pi;
-  uint64_t mask[2 * kMaxChunks];
+  uint64_t mask[2 * kMaxChunks];  // expand() builds them in lanes ch and 32 + ch
   uint32_t hdr;               // frame header, see hdr_* below
   uint8_t pos[20];            // element -> position
 };
@@ -446,13 +453,20 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
   }
   const uint32_t grdC = c.reorder ? (expC & 2u) : 0u;
   const uint64_t plainMask = c.reorder ? 0ULL : ~0ULL;
+  // the frame's masks collect in two registers, lane ch holding mask ch and
+  // lane 32 + ch mask nchunks + ch, and reach LDS in one store after the
+  // loop (a lane-0 store per chunk cost scalar instructions)
+  int mlo = 0, mhi = 0;
 
   // ---- candidates ------------------------------------------------------
   // software-pipelined: the next chunk's triple-table entries are read while
   // this chunk's cycle records are fetched
   int eNext = (int)lex[off + lane];
   uint64_t anyExp = 0;
-  for (int ch = 0; ch < nchunks; ++ch) {
+  // one exit: a witness ends the loop through its bound
+  int nend = nchunks;
+  bool wfound = false;
+  for (int ch = 0; ch < nend; ++ch) {
     const int t = (ch << 6) + lane;
     const int e = eNext;
     const int A = bperm(e & 31, einfo);
@@ -494,7 +508,8 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
         // every value it carries) as divergent
         c.wmove = uni(pi_ | (pj << 5) | (pk << 10));
         c.wlen = cd;
-        return kExpandFound;
+        wfound = true;
+        nend = 0;
       }
     }
     anyExp |= expb;
@@ -502,14 +517,18 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
     // frame in candidate order keeps them all in the first nchunks masks
     // (the second half is then never scanned).  No branch on the order.
     const uint64_t greedy = ballot(((grdC >> code) & 1u) != 0);
-    if (lane == 0) {
-      F.mask[ch] = greedy | (expb & plainMask);
-      F.mask[nchunks + ch] = expb & ~greedy;
-    }
+    const uint64_t lo = greedy | (expb & plainMask), hi = expb & ~greedy;
+    write_lane(mlo, (uint32_t)lo, ch);
+    write_lane(mhi, (uint32_t)(lo >> 32), ch);
+    write_lane(mlo, (uint32_t)hi, ch + 32);
+    write_lane(mhi, (uint32_t)(hi >> 32), ch + 32);
   }
+  if (wfound) return kExpandFound;
   // a node without expandable children leaves its frame unread: the search
   // resumes at the parent's
   if (anyExp == 0) return kExpandLeaf;
+  if ((lane & 31) < nchunks)
+    F.mask[lane < 32 ? lane : nchunks + lane - 32] = ((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo;
   if (lane == 0) {
     F.pi = pi;
     F.hdr = hdr_make(off, nchunks, full, c.reorder, 0);

@@ -558,25 +558,6 @@ static_assert(offsetof(Frame, hdr) % 8 == 0 && offsetof(Frame, pos) == offsetof(
                   sizeof(Frame) % 8 == 0 && kFrameWords <= 64,
               "run_item reads the frame by 8-byte word");
 
-// frame_move from a frame held one 8-byte word per lane (`fword`): position
-// of element e is byte e of the position map, which starts at the header
-// word's upper half.
-__device__ __forceinline__ int frame_pos(uint64_t fword, int e) {
-  const int b = e + 4;  // byte index from the header word's start
-  const uint64_t wv = (uint64_t)lane_value64((int64_t)fword, kHdrWord + (b >> 3));
-  return (int)((wv >> (8 * (b & 7))) & 0xFFu);
-}
-__device__ __forceinline__ int frame_move_words(uint64_t fword, uint32_t hdr, const uint16_t *lex,
-                                                int ch, int bit) {
-  const int e = uni(lex[hdr_off(hdr) + (hdr_cand_chunk(hdr, ch) << 6) + bit]);
-  int i = e & 31, j = (e >> 5) & 31, k = (e >> 10) & 31;
-  if (!hdr_full(hdr)) {
-    i = frame_pos(fword, i);
-    j = frame_pos(fword, j);
-    k = frame_pos(fword, k);
-  }
-  return i | (j << 5) | (k << 10);
-}
 
 // ---------------------------------------------------------------------------
 // Shared queue
@@ -922,7 +903,8 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
     // next expandable child: the first nonzero mask chunk at or after the
     // cursor (chunk q sits in lane q + 1)
     const int q = lane - 1;
-    const uint64_t nz = ballot(q >= hdr_cursor(hdr) && q < hdr_mask_chunks(hdr) && fword != 0);
+    const uint64_t nz = ballot((int)(q >= hdr_cursor(hdr)) & (int)(q < hdr_mask_chunks(hdr)) &
+                               (int)(fword != 0));
     if (nz == 0) {
       if (d == depth0) break;
       --d;
@@ -932,7 +914,18 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
     const int ch = chl - 1;
     const uint64_t bits = (uint64_t)lane_value64((int64_t)fword, chl);
     const int bit = __builtin_ctzll(bits);
-    const int mv = frame_move_words(fword, hdr, lex, ch, bit);
+    // the child's move: lanes 0..2 each look up one element's position in
+    // the frame's position map (one LDS read instead of a scalar chain of
+    // lane reads, shifts and masks per element)
+    int mv;
+    {
+      const int e = (int)lex[hdr_off(hdr) + (hdr_cand_chunk(hdr, ch) << 6) + bit];
+      const int sh = lane < 3 ? 5 * lane : 0;
+      const int el = (e >> sh) & 31;
+      const int part = (hdr_full(hdr) ? el : (int)F.pos[el]) << sh;
+      mv = __builtin_amdgcn_readlane(part, 0) | __builtin_amdgcn_readlane(part, 1) |
+           __builtin_amdgcn_readlane(part, 2);
+    }
     const uint64_t child = apply_move((uint64_t)lane_value64((int64_t)fword, 0), mv & 31,
                                       (mv >> 5) & 31, (mv >> 10) & 31);
     if (lane == 0) {

@@ -232,9 +232,10 @@ struct WaveLds {
 // arc between them closes on itself and the rest joins the third position's
 // cycle.  Three cycles merge into one.  Parities of the arcs: the distance
 // difference's, flipped when the arc wraps a cycle of odd length.  Code:
-// low two bits for the reference filter (0: delta 0, 1: +2, 2: -2, 3:
-// rejected: a three-cycle merge or a negative delta), the next two for the
-// full move set (never rejected).
+// delta + 2 (0, 2 or 4) in the low three bits for the reference filter, 6
+// when it rejects the move (a three-cycle merge or a negative delta), and
+// delta + 2 in the next three for the full move set (never rejected); the
+// code is a bit position in the per-node class masks of expand().
 constexpr int kDeltaCodes = 4096;
 __device__ __forceinline__ uint32_t delta_code(int idx) {
   const int la = idx & 1, qa = (idx >> 1) & 1, lb = (idx >> 2) & 1, qb = (idx >> 3) & 1;
@@ -253,15 +254,15 @@ __device__ __forceinline__ uint32_t delta_code(int idx) {
   } else {
     delta = (la ^ lb ^ lc) - la - lb - lc;
   }
-  const uint32_t cls = delta > 0 ? 1u : (delta < 0 ? 2u : 0u);
+  const uint32_t cls = (uint32_t)(delta + 2);
   const bool refOk = (sab | sbc | sac) && delta >= 0;
-  return (refOk ? cls : 3u) | (cls << 2);
+  return (refOk ? cls : 6u) | (cls << 3);
 }
 
 struct BlockLds {
   // odd-cycle delta codes of a triple, by its 12 record bits (delta_code);
-  // entry kDeltaCodes (and the padding) is code 3 in both move sets, read by
-  // lanes past a node's candidate count
+  // entry kDeltaCodes (and the padding) is code 7 (rejected) in both move
+  // sets, read by lanes past a node's candidate count
   uint8_t dtab[kDeltaCodes + 8];
   uint16_t lex[kLexEntries];
   // the candidate loop reads one chunk ahead, up to entry C(17,4) +
@@ -440,18 +441,14 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
   const int nchunks = (ncand + 63) >> 6;
   const int off = choose4(m);
   const int mode = c.flags & kFlagCountAll;
-  // code -> witness / expandable child / greedy child, as 4-bit tables
-  // (codes 0, 1, 2: delta 0, +2, -2; code 3: rejected)
-  const int codeShift = allMoves ? 2 : 0;
-  uint32_t satC = 0, expC = 0;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int cec = (ec + (k == 1 ? 2 : (k == 2 ? -2 : 0))) & 31;
-    const uint32_t s = (satMask >> cec) & 1u;
-    satC |= s << k;
-    expC |= (~s & (expMask >> cec) & 1u) << k;
-  }
-  const uint32_t grdC = c.reorder ? (expC & 2u) : 0u;
+  // code -> witness / expandable child / greedy child: bit delta + 2 of a
+  // class mask stands for a child with ec + delta odd cycles, i.e. bits
+  // ec - 2, ec and ec + 2 of the depth's masks shifted down to 0, 2 and 4
+  // (codes 6 and 7, rejected, hit no bit)
+  const int codeShift = allMoves ? 3 : 0;
+  const uint32_t satC = (uint32_t)(((uint64_t)satMask << 2) >> ec) & 0x15u;
+  const uint32_t expC = (uint32_t)(((uint64_t)expMask << 2) >> ec) & ~satC & 0x15u;
+  const uint32_t grdC = c.reorder ? (expC & 0x10u) : 0u;
   const uint64_t plainMask = c.reorder ? 0ULL : ~0ULL;
   // the frame's masks collect in two registers, lane ch holding mask ch and
   // lane 32 + ch mask nchunks + ch, and reach LDS in one store after the
@@ -482,10 +479,10 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
     idx |= (ua >> 24) < (ub >> 24) ? 512 : 0;
     idx |= (ub >> 24) < (uc >> 24) ? 1024 : 0;
     idx |= (uc >> 24) < (ua >> 24) ? 2048 : 0;
-    // lanes past the candidate count read code 3 and drop out of the ballots
+    // lanes past the candidate count read code 7 and drop out of the ballots
     // (a vector select: masking the ballots cost scalar instructions)
     if (lane >= ncand - (ch << 6)) idx = kDeltaCodes;
-    const uint32_t code = ((uint32_t)dtab[idx] >> codeShift) & 3u;
+    const uint32_t code = ((uint32_t)dtab[idx] >> codeShift) & 7u;
     const uint64_t satb = ballot(((satC >> code) & 1u) != 0);
     const uint64_t expb = ballot(((expC >> code) & 1u) != 0);
     if (satb != 0) {

@@ -168,11 +168,6 @@ __device__ __forceinline__ uint64_t apply_move(uint64_t pi, int i, int j, int k)
   return rest | (B >> (mid - lo)) | (A << (hi - mid));
 }
 
-__device__ __forceinline__ int choose3(int m) { return m < 3 ? 0 : m * (m - 1) * (m - 2) / 6; }
-__device__ __forceinline__ int choose4(int m) {
-  return m < 4 ? 0 : m * (m - 1) * (m - 2) * (m - 3) / 24;
-}
-
 // --------------------------------------------------------------------------
 // Per-wave LDS state
 // --------------------------------------------------------------------------
@@ -189,19 +184,19 @@ struct Frame {
 };
 
 // Frame header word: lexicographic table offset of the enumerated set
-// (C(m,4) <= 2380), candidate chunk count, "element q is position q" flag,
-// "reordered" flag and the first mask chunk that may still hold bits.  One
-// LDS word, read once per visit.
-__device__ __forceinline__ uint32_t hdr_make(int off, int nchunks, bool full, bool reordered,
-                                             int cursor) {
+// (C(m,4) <= 2380), candidate chunk count, "element q is position q" flag
+// and "reordered" flag; written once when the node is expanded.  A taken
+// child is cleared from its mask, so the frame's next child is always the
+// lowest bit of its first nonzero mask chunk (no cursor to keep: storing one
+// cost an LDS write per descent step).
+__device__ __forceinline__ uint32_t hdr_make(int off, int nchunks, bool full, bool reordered) {
   return (uint32_t)off | ((uint32_t)nchunks << 12) | (full ? 1u << 16 : 0u) |
-         (reordered ? 1u << 17 : 0u) | ((uint32_t)cursor << 24);
+         (reordered ? 1u << 17 : 0u);
 }
 __device__ __forceinline__ int hdr_off(uint32_t h) { return (int)(h & 0xFFFu); }
 __device__ __forceinline__ int hdr_nchunks(uint32_t h) { return (int)((h >> 12) & 15u); }
 __device__ __forceinline__ bool hdr_full(uint32_t h) { return (h >> 16) & 1u; }
 __device__ __forceinline__ bool hdr_reordered(uint32_t h) { return (h >> 17) & 1u; }
-__device__ __forceinline__ int hdr_cursor(uint32_t h) { return (int)(h >> 24); }
 // mask chunks to scan, and the candidate chunk a mask chunk stands for
 __device__ __forceinline__ int hdr_mask_chunks(uint32_t h) {
   return hdr_reordered(h) ? 2 * hdr_nchunks(h) : hdr_nchunks(h);
@@ -434,12 +429,13 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
   const int einfoB = einfo + ((einfo & 0x300) * 3);
   const int einfoC = einfo + ((einfo & 0x300) * 15);
   m = uni(m);
-  const int ncand = choose3(m);
+  // C(m,3) and C(m,4) in unsigned arithmetic (both vanish below 3 and 4)
+  const int ncand = (int)((uint32_t)(m * (m - 1) * (m - 2)) / 6u);
   if (kCount && lane == 0)
     __hip_atomic_fetch_add(cands, (uint64_t)ncand, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WAVEFRONT);
   const int nchunks = (ncand + 63) >> 6;
-  const int off = choose4(m);
+  const int off = (int)((uint32_t)(ncand * (m - 3)) >> 2);
   const int mode = c.flags & kFlagCountAll;
   // code -> witness / expandable child / greedy child: bit delta + 2 of a
   // class mask stands for a child with ec + delta odd cycles, i.e. bits
@@ -528,7 +524,7 @@ __device__ int expand(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex,
     F.mask[lane < 32 ? lane : nchunks + lane - 32] = ((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo;
   if (lane == 0) {
     F.pi = pi;
-    F.hdr = hdr_make(off, nchunks, full, c.reorder, 0);
+    F.hdr = hdr_make(off, nchunks, full, c.reorder);
   }
   wave_sync();
   return kExpandOk;
@@ -581,7 +577,7 @@ __device__ int donate(const Params &P, Ctx &c, WaveLds &w, const uint16_t *lex, 
   for (int q = c.depth0; q <= d && q <= maxFd; ++q) {
     const uint32_t h = (uint32_t)uni((int)w.frames[q].hdr);
     const int nch = hdr_mask_chunks(h);
-    for (int x = hdr_cursor(h); x < nch; ++x) {
+    for (int x = 0; x < nch; ++x) {
       if (uni64(w.frames[q].mask[x]) != 0) {
         fd = q;
         ch = x;
@@ -897,11 +893,11 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
     Frame &F = w.frames[d];
     const uint64_t fword = reinterpret_cast<const uint64_t *>(&F)[fq];
     const uint32_t hdr = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)fword, kHdrWord);
-    // next expandable child: the first nonzero mask chunk at or after the
-    // cursor (chunk q sits in lane q + 1)
+    // next expandable child: the first nonzero mask chunk (chunk q sits in
+    // lane q + 1)
     const int q = lane - 1;
-    const uint64_t nz = ballot((int)(q >= hdr_cursor(hdr)) & (int)(q < hdr_mask_chunks(hdr)) &
-                               (int)(fword != 0));
+    const uint64_t nz =
+        ballot((int)((uint32_t)q < (uint32_t)hdr_mask_chunks(hdr)) & (int)(fword != 0));
     if (nz == 0) {
       if (d == depth0) break;
       --d;
@@ -927,7 +923,6 @@ __device__ bool run_item(const Params &P, Ctx &c, WaveLds &w, const uint16_t *le
                                       (mv >> 5) & 31, (mv >> 10) & 31);
     if (lane == 0) {
       F.mask[ch] = bits & (bits - 1);
-      F.hdr = (hdr & 0x00FFFFFFu) | ((uint32_t)ch << 24);
       w.path[d] = (uint16_t)mv;
     }
     wave_sync();
